@@ -1,0 +1,151 @@
+"""ctypes binding of libfvdbm.so (the C ABI declared in include/fvdbm.h).
+
+The library is loaded from fvdbm-jax_amd/_build/ and nothing else: there is
+no CPU fallback. A missing library raises ``ImportError`` naming the build
+step. Before loading it, the HIP runtime bundled with PyTorch-ROCm is loaded
+by path (when PyTorch is installed), so the solver and any PyTorch code in
+the process share one HIP runtime whichever is imported first.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import importlib.util
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfvdbm.so")
+
+FP32, FP64 = 0, 1
+UPWIND, LAX_WENDROFF = 0, 1
+CELL_PDF, CELL_RHO, CELL_VEL, CELL_PDF_EQ, FACE_FLUX, NODE_PDF, NODE_RHO, NODE_VEL = range(8)
+ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_STATE = 1, 2, 3, 4
+
+
+class MeshDesc(ct.Structure):
+    _fields_ = [("n_cells", ct.c_int32), ("n_faces", ct.c_int32),
+                ("n_nodes", ct.c_int32), ("k_max", ct.c_int32),
+                ("cell_faces", ct.c_void_p), ("cell_signs", ct.c_void_p),
+                ("face_cells", ct.c_void_p), ("face_dists", ct.c_void_p),
+                ("face_nodes", ct.c_void_p), ("face_n", ct.c_void_p),
+                ("face_L", ct.c_void_p), ("node_cells", ct.c_void_p),
+                ("node_dists", ct.c_void_p), ("node_type", ct.c_void_p),
+                ("node_rho", ct.c_void_p), ("node_vel", ct.c_void_p)]
+
+
+class Params(ct.Structure):
+    _fields_ = [("tau", ct.c_double), ("delta_t", ct.c_double),
+                ("precision", ct.c_int32), ("scheme", ct.c_int32),
+                ("reorder", ct.c_int32), ("variant", ct.c_int32),
+                ("block", ct.c_int32), ("graph_steps", ct.c_int32)]
+
+
+class Info(ct.Structure):
+    _fields_ = [(n, ct.c_int32) for n in (
+        "n_cells", "n_faces", "n_nodes", "k_max", "n_cells_padded",
+        "n_boundary_cells", "n_boundary_faces", "n_active_nodes", "bandwidth",
+        "precision", "scheme", "variant", "block", "graph_steps")] + [
+        ("steps_done", ct.c_int64), ("device_bytes", ct.c_int64)]
+
+
+# every function the header declares, with its ctypes signature
+_SIGNATURES = {
+    "fvdbm_last_error": (ct.c_char_p, []),
+    "fvdbm_abi_version": (ct.c_int, []),
+    "fvdbm_device_count": (ct.c_int, [ct.POINTER(ct.c_int32)]),
+    "fvdbm_variant_count": (ct.c_int, [ct.POINTER(ct.c_int32)]),
+    "fvdbm_create": (ct.c_int, [ct.POINTER(MeshDesc), ct.POINTER(Params), ct.POINTER(ct.c_void_p)]),
+    "fvdbm_destroy": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_info_get": (ct.c_int, [ct.c_void_p, ct.POINTER(Info)]),
+    "fvdbm_init_uniform": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_set_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),
+    "fvdbm_get_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),
+    "fvdbm_step": (ct.c_int, [ct.c_void_p, ct.c_int64]),
+    "fvdbm_step_residual": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
+    "fvdbm_step_timed": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
+    "fvdbm_synchronize": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "fvdbm_layout_order": (ct.c_int, [ct.POINTER(MeshDesc), ct.c_int32, ct.c_int32,
+                                      ct.c_void_p, ct.c_void_p]),
+}
+
+_lib = None
+
+
+def _preload_torch_hip_runtime():
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        return ct.CDLL(path, mode=ct.RTLD_GLOBAL)
+    return None
+
+
+def lib():
+    """The loaded library (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` "
+                "(the solver has no CPU fallback)")
+        _preload_torch_hip_runtime()
+        L = ct.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGNATURES)
+
+
+class FvdbmError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    """Raise for a non-zero status: ValueError for invalid input (the
+    reference's convention), RuntimeError otherwise."""
+    if rc == 0:
+        return
+    msg = lib().fvdbm_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == ERR_INVALID:
+        raise ValueError(text)
+    raise FvdbmError(f"{text} (status {rc})")
+
+
+def device_count():
+    n = ct.c_int32(0)
+    check(lib().fvdbm_device_count(ct.byref(n)), "fvdbm_device_count")
+    return int(n.value)
+
+
+def ptr(a):
+    return ct.c_void_p(a.ctypes.data) if a is not None else ct.c_void_p(0)
+
+
+def mesh_desc(arrays, dtype):
+    """MeshDesc over a dict of contiguous arrays (kept alive by the caller).
+    Keys: cell_faces, cell_signs, face_cells, face_dists, face_nodes, face_n,
+    face_L, node_cells, node_dists, node_type, node_rho, node_vel."""
+    a = arrays
+    for k in ("cell_faces", "cell_signs", "face_cells", "face_nodes", "node_cells", "node_type"):
+        a[k] = np.ascontiguousarray(a[k], dtype=np.int32)
+    for k in ("face_dists", "face_n", "face_L", "node_dists", "node_rho", "node_vel"):
+        a[k] = np.ascontiguousarray(a[k], dtype=dtype)
+    C = a["cell_faces"].shape[0]
+    F = a["face_nodes"].shape[0]
+    N = a["node_type"].reshape(-1).shape[0]
+    K = a["node_cells"].shape[1] if a["node_cells"].ndim == 2 else 0
+    return MeshDesc(C, F, N, K, *[ptr(a[k]) for k in (
+        "cell_faces", "cell_signs", "face_cells", "face_dists", "face_nodes",
+        "face_n", "face_L", "node_cells", "node_dists", "node_type",
+        "node_rho", "node_vel")])

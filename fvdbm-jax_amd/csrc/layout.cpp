@@ -1,0 +1,377 @@
+// Host-side device-layout construction (see layout.h).
+#include "layout.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace fvdbm {
+
+namespace {
+
+// BFS over the unvisited component of `s`; returns the last level and the
+// eccentricity of s. `stamp`/`mark` avoid clearing a per-call array.
+int bfs_last_level(int s, const std::vector<int32_t> &xadj,
+                   const std::vector<int32_t> &adj, const std::vector<char> &placed,
+                   std::vector<int32_t> &mark, int32_t stamp,
+                   std::vector<int32_t> &queue, std::vector<int32_t> &last) {
+  queue.clear();
+  queue.push_back(s);
+  mark[s] = stamp;
+  int depth = 0;
+  size_t level_start = 0, level_end = 1;
+  for (;;) {
+    for (size_t h = level_start; h < level_end; ++h) {
+      int v = queue[h];
+      for (int e = xadj[v]; e < xadj[v + 1]; ++e) {
+        int u = adj[e];
+        if (placed[u] || mark[u] == stamp) continue;
+        mark[u] = stamp;
+        queue.push_back(u);
+      }
+    }
+    if (queue.size() == level_end) break;
+    level_start = level_end;
+    level_end = queue.size();
+    ++depth;
+  }
+  last.assign(queue.begin() + level_start, queue.begin() + level_end);
+  return depth;
+}
+
+}  // namespace
+
+std::vector<int32_t> reverse_cuthill_mckee(int n, const std::vector<int32_t> &xadj,
+                                           const std::vector<int32_t> &adj) {
+  std::vector<int32_t> deg(n);
+  for (int i = 0; i < n; ++i) deg[i] = xadj[i + 1] - xadj[i];
+  std::vector<char> placed(n, 0);
+  std::vector<int32_t> mark(n, -1), queue, last, order, nbrs;
+  order.reserve(n);
+  int32_t stamp = 0;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (placed[s0]) continue;
+    // George-Liu pseudo-peripheral node: restart from the lowest-degree node
+    // of the last BFS level while the eccentricity grows.
+    int root = s0;
+    int ecc = bfs_last_level(root, xadj, adj, placed, mark, stamp++, queue, last);
+    for (int it = 0; it < 16; ++it) {
+      int best = last[0];
+      for (int v : last)
+        if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+      int e2 = bfs_last_level(best, xadj, adj, placed, mark, stamp++, queue, last);
+      if (e2 <= ecc) break;
+      ecc = e2;
+      root = best;
+    }
+    // Cuthill-McKee: BFS, unplaced neighbours in increasing (degree, id).
+    size_t head = order.size();
+    order.push_back(root);
+    placed[root] = 1;
+    while (head < order.size()) {
+      int v = order[head++];
+      nbrs.clear();
+      for (int e = xadj[v]; e < xadj[v + 1]; ++e) {
+        int u = adj[e];
+        if (!placed[u]) {
+          placed[u] = 1;
+          nbrs.push_back(u);
+        }
+      }
+      std::sort(nbrs.begin(), nbrs.end(), [&](int a, int b) {
+        return deg[a] < deg[b] || (deg[a] == deg[b] && a < b);
+      });
+      order.insert(order.end(), nbrs.begin(), nbrs.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+template <typename T>
+bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
+                  HostLayout<T> &L, std::string &err) {
+  const int C = m.n_cells, F = m.n_faces, N = m.n_nodes, K = m.k_max;
+  if (C <= 0 || F <= 0 || N <= 0 || K < 0) {
+    err = "mesh must have cells, faces and nodes";
+    return false;
+  }
+  if (!m.cell_faces || !m.cell_signs || !m.face_cells || !m.face_dists ||
+      !m.face_nodes || !m.face_n || !m.face_L || !m.node_type ||
+      (K > 0 && (!m.node_cells || !m.node_dists)) || !m.node_rho || !m.node_vel) {
+    err = "mesh descriptor has a null array";
+    return false;
+  }
+  if (C > kIndexMask || F > kIndexMask || N > kIndexMask) {
+    err = "mesh too large for 30-bit indices";
+    return false;
+  }
+  const T *fd = static_cast<const T *>(m.face_dists);
+  const T *fn = static_cast<const T *>(m.face_n);
+  const T *fL = static_cast<const T *>(m.face_L);
+  const T *nd = static_cast<const T *>(m.node_dists);
+  const T *nrho = static_cast<const T *>(m.node_rho);
+  const T *nvel = static_cast<const T *>(m.node_vel);
+
+  L.C = C;
+  L.F = F;
+  L.N = N;
+  L.K = K;
+
+  // --- per cell-face: slot of the cell in the face stencil and the other side
+  std::vector<int32_t> other(3 * (size_t)C), slot(3 * (size_t)C);
+  std::vector<char> has_ghost(C, 0);
+  for (int c = 0; c < C; ++c) {
+    for (int k = 0; k < 3; ++k) {
+      int fid = m.cell_faces[3 * c + k];
+      if (fid < 0 || fid >= F) {
+        err = "cells.face_indices out of range at cell " + std::to_string(c);
+        return false;
+      }
+      int s = m.cell_signs[3 * c + k];
+      if (s < -1 || s > 1) {
+        err = "cells.face_normals must hold signs in {-1,0,1}";
+        return false;
+      }
+      int a = m.face_cells[2 * fid], b = m.face_cells[2 * fid + 1];
+      if (a < -1 || a >= C || b < -1 || b >= C) {
+        err = "faces.stencil_cells_index out of range at face " + std::to_string(fid);
+        return false;
+      }
+      int sl, o;
+      if (a == c && b != c) {
+        sl = 0;
+        o = b;
+      } else if (b == c && a != c) {
+        sl = 1;
+        o = a;
+      } else {
+        err = "cell " + std::to_string(c) + " is not in the stencil of its face " +
+              std::to_string(fid);
+        return false;
+      }
+      slot[3 * c + k] = sl;
+      other[3 * c + k] = o;
+      if (o == -1) {
+        has_ghost[c] = 1;
+        int p = m.face_nodes[2 * fid], r = m.face_nodes[2 * fid + 1];
+        if (p < 0 || p >= N || r < 0 || r >= N) {
+          err = "faces.nodes_index out of range at face " + std::to_string(fid);
+          return false;
+        }
+      }
+    }
+  }
+
+  // --- cell order
+  std::vector<int32_t> perm(C);
+  if (reorder == 0) {
+    std::iota(perm.begin(), perm.end(), 0);
+  } else {
+    std::vector<int32_t> xadj(C + 1, 0), adj;
+    adj.reserve(3 * (size_t)C);
+    for (int c = 0; c < C; ++c) {
+      for (int k = 0; k < 3; ++k)
+        if (other[3 * c + k] >= 0) adj.push_back(other[3 * c + k]);
+      xadj[c + 1] = (int32_t)adj.size();
+    }
+    perm = reverse_cuthill_mckee(C, xadj, adj);
+    if (reorder == 1) {
+      std::stable_partition(perm.begin(), perm.end(),
+                            [&](int32_t c) { return has_ghost[c] != 0; });
+    }
+  }
+  L.new_to_old = perm;
+  L.old_to_new.assign(C, 0);
+  for (int i = 0; i < C; ++i) L.old_to_new[perm[i]] = i;
+  L.n_bcells = 0;
+  for (int c = 0; c < C; ++c) L.n_bcells += has_ghost[c];
+  L.last_cell_new = L.old_to_new[C - 1];
+  const int pad = tile > 0 ? tile : 1;
+  L.Cp = ((C + pad - 1) / pad) * pad;
+  const size_t Cp = (size_t)L.Cp;
+
+  // --- active nodes
+  std::vector<int32_t> node_an(N, -1);
+  for (int i = 0; i < N; ++i) {
+    int t = m.node_type[i];
+    if (t == 1 || t == 2) {
+      node_an[i] = (int32_t)L.an_gid.size();
+      L.an_gid.push_back(i);
+      L.an_type.push_back(t);
+    } else if (t != 0) {
+      err = "nodes.type must be 0, 1 or 2";
+      return false;
+    }
+  }
+  L.nan = (int)L.an_gid.size();
+  L.an_cells.assign((size_t)L.nan * K, 0);
+  L.an_w.assign((size_t)L.nan * K, T(0));
+  L.an_wsum.assign(L.nan, T(0));
+  L.an_rho.resize(L.nan);
+  L.an_ux.resize(L.nan);
+  L.an_uy.resize(L.nan);
+  for (int a = 0; a < L.nan; ++a) {
+    int i = L.an_gid[a];
+    T wsum = T(0);
+    for (int k = 0; k < K; ++k) {
+      int ci = m.node_cells[(size_t)i * K + k];
+      T d = nd[(size_t)i * K + k];
+      // utils.extrapolate: w = 1/d, negative weights -> 0 (utils/utils.py:58-59)
+      T w = T(1) / d;
+      if (w < T(0)) w = T(0);
+      if (ci < -1 || ci >= C) {
+        err = "nodes.cells_index out of range at node " + std::to_string(i);
+        return false;
+      }
+      L.an_cells[(size_t)a * K + k] = ci >= 0 ? L.old_to_new[ci] : -2 - L.last_cell_new;
+      L.an_w[(size_t)a * K + k] = w;
+      wsum += w;
+    }
+    L.an_wsum[a] = wsum;
+    L.an_rho[a] = nrho[i];
+    L.an_ux[a] = nvel[2 * (size_t)i];
+    L.an_uy[a] = nvel[2 * (size_t)i + 1];
+  }
+
+  // --- per cell-face arrays in device order
+  L.nb.assign(3 * Cp, 0);
+  L.nx.assign(3 * Cp, T(0));
+  L.ny.assign(3 * Cp, T(0));
+  L.sL.assign(3 * Cp, T(0));
+  L.lwa.assign(3 * Cp, T(0));
+  L.lwden2.assign(3 * Cp, T(1));
+  L.bf_ratio.clear();
+  std::vector<int32_t> bf0, bf1, bs0, bs1;
+  L.ntiles = L.Cp / pad;
+  L.tile_node_off.assign(L.ntiles + 1, 0);
+  L.tile_nodes.clear();
+  std::vector<int32_t> slot_of(N, -1);  // node -> slot in the current tile table
+  std::vector<int32_t> touched;
+  int cur_tile = -1;
+  auto tile_slot = [&](int node) {
+    if (slot_of[node] < 0) {
+      slot_of[node] = (int32_t)(L.tile_nodes.size() - L.tile_node_off[cur_tile]);
+      L.tile_nodes.push_back(node_an[node] >= 0 ? node_an[node] : -1 - node);
+      touched.push_back(node);
+    }
+    return slot_of[node];
+  };
+  int bw = 0;
+  for (int i = 0; i < C; ++i) {
+    const int t = i / pad;
+    while (cur_tile < t) {  // close the previous tile's node table
+      for (int nd_ : touched) slot_of[nd_] = -1;
+      touched.clear();
+      ++cur_tile;
+      L.tile_node_off[cur_tile] = (int32_t)L.tile_nodes.size();
+    }
+    int c = perm[i];
+    for (int k = 0; k < 3; ++k) {
+      int fid = m.cell_faces[3 * c + k];
+      int sl = slot[3 * c + k], o = other[3 * c + k];
+      size_t at = (size_t)k * Cp + i;
+      T d0 = fd[2 * (size_t)fid], d1 = fd[2 * (size_t)fid + 1];
+      L.nx[at] = fn[2 * (size_t)fid];
+      L.ny[at] = fn[2 * (size_t)fid + 1];
+      L.sL[at] = T(m.cell_signs[3 * c + k]) * fL[fid];
+      T s = d0 + d1;
+      L.lwa[at] = d0 / s;
+      L.lwden2[at] = T(2) * s;
+      int32_t code;
+      if (o >= 0) {
+        int j = L.old_to_new[o];
+        code = j;
+        bw = std::max(bw, std::abs(j - i));
+      } else {
+        int bf = (int)L.bf_ratio.size();
+        // ghost distance / known distance: (d1/d0) when the cell is slot 0
+        L.bf_ratio.push_back(sl == 0 ? d1 / d0 : d0 / d1);
+        int p = m.face_nodes[2 * (size_t)fid], r = m.face_nodes[2 * (size_t)fid + 1];
+        bf0.push_back(p);
+        bf1.push_back(r);
+        bs0.push_back(tile_slot(p));
+        bs1.push_back(tile_slot(r));
+        code = kGhostBit | bf;
+      }
+      if (sl == 1) code |= kSlot1Bit;
+      L.nb[at] = code;
+    }
+  }
+  while (cur_tile < L.ntiles - 1) {
+    ++cur_tile;
+    L.tile_node_off[cur_tile] = (int32_t)L.tile_nodes.size();
+  }
+  L.tile_node_off[L.ntiles] = (int32_t)L.tile_nodes.size();
+  L.max_tile_nodes = 0;
+  for (int t = 0; t < L.ntiles; ++t)
+    L.max_tile_nodes = std::max(L.max_tile_nodes, L.tile_node_off[t + 1] - L.tile_node_off[t]);
+  L.nbf = (int)L.bf_ratio.size();
+  L.bf_gnode.resize(2 * (size_t)L.nbf);
+  L.bf_slot.resize(2 * (size_t)L.nbf);
+  std::copy(bf0.begin(), bf0.end(), L.bf_gnode.begin());
+  std::copy(bf1.begin(), bf1.end(), L.bf_gnode.begin() + L.nbf);
+  std::copy(bs0.begin(), bs0.end(), L.bf_slot.begin());
+  std::copy(bs1.begin(), bs1.end(), L.bf_slot.begin() + L.nbf);
+  L.bandwidth = bw;
+
+  // --- faces in reference order (for the flux observable)
+  L.fc0.resize(F);
+  L.fc1.resize(F);
+  L.fn0.resize(F);
+  L.fn1.resize(F);
+  L.fd0.resize(F);
+  L.fd1.resize(F);
+  L.fnx.resize(F);
+  L.fny.resize(F);
+  L.fL.resize(F);
+  for (int j = 0; j < F; ++j) {
+    int a = m.face_cells[2 * j], b = m.face_cells[2 * j + 1];
+    if (a < -1 || a >= C || b < -1 || b >= C) {
+      err = "faces.stencil_cells_index out of range at face " + std::to_string(j);
+      return false;
+    }
+    int p = m.face_nodes[2 * j], r = m.face_nodes[2 * j + 1];
+    if (p < 0 || p >= N || r < 0 || r >= N) {
+      err = "faces.nodes_index out of range at face " + std::to_string(j);
+      return false;
+    }
+    L.fc0[j] = a >= 0 ? L.old_to_new[a] : -1;
+    L.fc1[j] = b >= 0 ? L.old_to_new[b] : -1;
+    L.fn0[j] = p;
+    L.fn1[j] = r;
+    L.fd0[j] = fd[2 * (size_t)j];
+    L.fd1[j] = fd[2 * (size_t)j + 1];
+    L.fnx[j] = fn[2 * (size_t)j];
+    L.fny[j] = fn[2 * (size_t)j + 1];
+    L.fL[j] = fL[j];
+  }
+  return true;
+}
+
+template bool build_layout<float>(const fvdbm_mesh_desc &, int, int, HostLayout<float> &,
+                                  std::string &);
+template bool build_layout<double>(const fvdbm_mesh_desc &, int, int, HostLayout<double> &,
+                                   std::string &);
+
+}  // namespace fvdbm
+
+// Host-only entry point (no device needed): the device cell order and layout
+// statistics the library would use for this mesh.
+extern "C" int fvdbm_layout_order(const fvdbm_mesh_desc *mesh, int32_t precision,
+                                  int32_t reorder, int32_t *new_to_old, int32_t *stats) {
+  if (!mesh || !new_to_old || !stats) return FVDBM_ERR_INVALID;
+  std::string err;
+  if (precision == FVDBM_FP64) {
+    fvdbm::HostLayout<double> L;
+    if (!fvdbm::build_layout<double>(*mesh, reorder, 1, L, err)) return FVDBM_ERR_INVALID;
+    std::copy(L.new_to_old.begin(), L.new_to_old.end(), new_to_old);
+    stats[0] = L.n_bcells; stats[1] = L.nbf; stats[2] = L.nan; stats[3] = L.bandwidth;
+  } else {
+    fvdbm::HostLayout<float> L;
+    if (!fvdbm::build_layout<float>(*mesh, reorder, 1, L, err)) return FVDBM_ERR_INVALID;
+    std::copy(L.new_to_old.begin(), L.new_to_old.end(), new_to_old);
+    stats[0] = L.n_bcells; stats[1] = L.nbf; stats[2] = L.nan; stats[3] = L.bandwidth;
+  }
+  return FVDBM_OK;
+}

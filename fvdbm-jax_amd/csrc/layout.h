@@ -1,0 +1,72 @@
+// Host-side construction of the device layout: cell renumbering (reverse
+// Cuthill-McKee, cells with a ghost face first), per cell-face structure of
+// arrays, boundary-face and active-node records. Pure C++ (no HIP calls), so
+// it is also exercised by the CPU tests through fvdbm_layout_* helpers.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fvdbm.h"
+
+namespace fvdbm {
+
+// Bits of a cell-face code (nb array):
+//   bit 31 set      : ghost face; bits 0..29 = boundary-face record index
+//   bit 31 clear    : interior face; bits 0..29 = neighbour cell (device id)
+//   bit 30 set      : this cell sits in stencil slot 1 of the face
+constexpr int32_t kGhostBit = int32_t(0x80000000u);
+constexpr int32_t kSlot1Bit = 0x40000000;
+constexpr int32_t kIndexMask = 0x3fffffff;
+
+template <typename T>
+struct HostLayout {
+  int C = 0, Cp = 0, F = 0, N = 0, K = 0;
+  int n_bcells = 0;   // cells with >= 1 ghost face, stored first when reorder==1
+  int nbf = 0;        // boundary-face records
+  int nan = 0;        // active (type 1 / 2) nodes
+  int bandwidth = 0;
+  int last_cell_new = 0;  // device id of reference cell C-1 (JAX's index -1)
+  std::vector<int32_t> new_to_old, old_to_new;
+
+  // per cell-face, [3][Cp]
+  std::vector<int32_t> nb;
+  std::vector<T> nx, ny, sL;      // face normal, sign * L
+  std::vector<T> lwa, lwden2;     // Lax-Wendroff: d0/(d0+d1), 2*(d0+d1)
+
+  // boundary-face records: [2][nbf] endpoint slots in the owning tile's
+  // node table, [2][nbf] endpoint node ids, [nbf] ghost/known distance ratio
+  std::vector<int32_t> bf_slot, bf_gnode;
+  std::vector<T> bf_ratio;
+
+  // per-tile node tables (tile = `block` consecutive device cells): the
+  // distinct face nodes the tile's ghost faces read, CSR over tiles.
+  // entry >= 0: active-node index; < 0: -1 - node id (stored pdf)
+  int ntiles = 0, max_tile_nodes = 0;
+  std::vector<int32_t> tile_node_off, tile_nodes;
+
+  // active nodes
+  std::vector<int32_t> an_gid, an_type;
+  std::vector<int32_t> an_cells;  // [nan][K]: device id, or -2 - (wrapped id) for padding
+  std::vector<T> an_w;            // [nan][K]  1/d, negatives -> 0
+  std::vector<T> an_wsum;         // [nan]     0-initialised ordered sum of an_w
+  std::vector<T> an_rho, an_ux, an_uy;
+
+  // faces in reference order (flux observable)
+  std::vector<int32_t> fc0, fc1, fn0, fn1;
+  std::vector<T> fd0, fd1, fnx, fny, fL;
+};
+
+// `tile` is the number of cells per thread block; the cell count is padded
+// to a multiple of it.
+template <typename T>
+bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
+                  HostLayout<T> &out, std::string &err);
+
+// Reverse Cuthill-McKee over an undirected graph in CSR form.
+// Returns perm with perm[new] = old.
+std::vector<int32_t> reverse_cuthill_mckee(int n, const std::vector<int32_t> &xadj,
+                                           const std::vector<int32_t> &adj);
+
+}  // namespace fvdbm

@@ -1,0 +1,225 @@
+"""Environment: the drop-in for the reference's ``Environment``
+(src/environment.py:10-67).
+
+``Environment(cells, faces, nodes)`` then ``.init()`` uploads the static
+geometry once to a device handle (libfvdbm); ``env = env.step()`` advances
+one step on the GPU and returns ``self``, matching the reference idiom
+``for i in range(n): env = env.step()`` (README.md:99-106). After a step,
+``cells.pdf`` is the new pdf while ``cells.rho``/``vel``/``pdf_eq``,
+``faces.pdf`` and the node fields describe the start of that step, exactly
+as the reference leaves them (src/environment.py:60-64).
+
+Differences from the reference, by design:
+  * state lives on the device; attribute reads copy to the host lazily;
+  * ``step(nsteps)`` accepts a step count (default 1) and runs the steps as
+    captured hipGraphs; ``step_residual`` and ``run_to_steady`` add the
+    convergence residual the reference lacks;
+  * ``precision="fp64"`` runs the same kernels in double precision (the
+    FP64 twin used to state the FP32 error).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from .containers import Cells, Faces, Nodes
+
+_FIELDS = {
+    ("cells", "pdf"): (_lib.CELL_PDF, 9),
+    ("cells", "rho"): (_lib.CELL_RHO, 1),
+    ("cells", "vel"): (_lib.CELL_VEL, 2),
+    ("cells", "pdf_eq"): (_lib.CELL_PDF_EQ, 9),
+    ("faces", "pdf"): (_lib.FACE_FLUX, 9),
+    ("nodes", "pdf"): (_lib.NODE_PDF, 9),
+    ("nodes", "rho"): (_lib.NODE_RHO, 1),
+    ("nodes", "vel"): (_lib.NODE_VEL, 2),
+}
+_SETTABLE = {("cells", "pdf"), ("nodes", "pdf"), ("nodes", "rho"), ("nodes", "vel")}
+
+
+def _f64_or(obj, name, fallback):
+    v = getattr(obj, name, None)
+    return np.asarray(fallback, np.float64) if v is None else np.asarray(v, np.float64)
+
+
+class Environment:
+    """Cells, faces and nodes of one mesh plus the device state that steps them."""
+
+    dynamics = None  # class attribute used by create(), as in the reference
+
+    def __init__(self, cells: Cells, faces: Faces, nodes: Nodes, precision="fp32",
+                 reorder=1, variant=0, block=0, graph_steps=0):
+        self.cells = cells
+        self.faces = faces
+        self.nodes = nodes
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        self.precision = precision
+        self.options = dict(reorder=reorder, variant=variant, block=block,
+                            graph_steps=graph_steps)
+        self._h = None
+
+    @classmethod
+    def create(cls, num_cells, num_faces, num_nodes):
+        """Empty containers to fill by hand (src/environment.py:20-27)."""
+        temp = cls.__new__(cls)
+        Environment.__init__(temp, Cells(num_cells, cls.dynamics),
+                             Faces(num_faces, cls.dynamics), Nodes(num_nodes, cls.dynamics))
+        return temp
+
+    @classmethod
+    def define(cls, cells, faces, nodes):
+        temp = cls.__new__(cls)
+        Environment.__init__(temp, cells, faces, nodes)
+        return temp
+
+    # ------------------------------------------------------------------
+    @property
+    def dtype(self):
+        return np.float64 if self.precision == "fp64" else np.float32
+
+    def _mesh_arrays(self):
+        c, f, n = self.cells, self.faces, self.nodes
+        N = n.size
+        if self.precision == "fp64":
+            face_dists = _f64_or(f, "stencil_dists_f64", f.stencil_dists)
+            face_n = _f64_or(f, "n_f64", f.n)
+            face_L = _f64_or(f, "L_f64", f.L)
+            node_dists = _f64_or(n, "cell_dists_f64", n.cell_dists)
+            node_rho = _f64_or(n, "rho_f64", n.__dict__["_host_rho"])
+            node_vel = _f64_or(n, "vel_f64", n.__dict__["_host_vel"])
+        else:
+            face_dists, face_n, face_L = f.stencil_dists, f.n, f.L
+            node_dists = n.cell_dists
+            node_rho = n.__dict__["_host_rho"]
+            node_vel = n.__dict__["_host_vel"]
+        return dict(
+            cell_faces=c.face_indices, cell_signs=c.face_normals,
+            face_cells=f.stencil_cells_index, face_dists=face_dists,
+            face_nodes=f.nodes_index, face_n=face_n,
+            face_L=np.asarray(face_L).reshape(-1), node_cells=n.cells_index,
+            node_dists=node_dists, node_type=np.asarray(n.type).reshape(N),
+            node_rho=np.asarray(node_rho).reshape(N), node_vel=np.asarray(node_vel).reshape(N, 2))
+
+    def init(self):
+        """Convert the ragged arrays, upload the static geometry and the
+        initial state (src/environment.py:38-42)."""
+        if self.faces.flux_scheme not in ("upwind", "lax_wendroff"):
+            raise ValueError(f"Unknown flux scheme: {self.faces.flux_scheme}")
+        self.cells.init()
+        self.faces.init()
+        self.nodes.init()
+        self.close()
+        dt = self.dtype
+        arrays = self._mesh_arrays()
+        desc = _lib.mesh_desc(arrays, dt)
+        dyn = self.cells.dynamics
+        p = _lib.Params(float(dyn.tau), float(dyn.delta_t),
+                        _lib.FP64 if self.precision == "fp64" else _lib.FP32,
+                        _lib.LAX_WENDROFF if self.faces.flux_scheme == "lax_wendroff" else _lib.UPWIND,
+                        int(self.options["reorder"]), int(self.options["variant"]),
+                        int(self.options["block"]), int(self.options["graph_steps"]))
+        h = ct.c_void_p()
+        _lib.check(_lib.lib().fvdbm_create(ct.byref(desc), ct.byref(p), ct.byref(h)), "fvdbm_create")
+        self._h = h
+        self._arrays = arrays  # keep host copies alive for the handle's lifetime
+        # initial state: fvdbm_create set every pdf to W in the working
+        # precision; upload only pdfs the caller changed before init()
+        W32 = np.asarray(dyn.W, np.float32)[None, :]
+        for field, obj in ((_lib.CELL_PDF, self.cells), (_lib.NODE_PDF, self.nodes)):
+            host = np.asarray(obj.__dict__["_host_pdf"])
+            if not (host.dtype == np.float32 and np.array_equal(host, np.broadcast_to(W32, host.shape))):
+                self._upload(field, host, 9)
+        for obj in (self.cells, self.faces, self.nodes):
+            obj.__dict__["_env"] = self
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().fvdbm_destroy(self._h)
+            self._h = None
+            for obj in (self.cells, self.faces, self.nodes):
+                obj.__dict__["_env"] = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _handle(self):
+        if not self._h:
+            raise RuntimeError("Environment.init() has not been called")
+        return self._h
+
+    def _upload(self, field, value, comps):
+        arr = np.ascontiguousarray(np.asarray(value, dtype=self.dtype).reshape(-1, comps))
+        _lib.check(_lib.lib().fvdbm_set_field(self._handle(), field, _lib.ptr(arr)), "fvdbm_set_field")
+
+    # container attribute plumbing ---------------------------------------
+    def _read(self, kind, name):
+        field, comps = _FIELDS[(kind, name)]
+        n = {"cells": self.cells, "faces": self.faces, "nodes": self.nodes}[kind].size
+        out = np.empty((n, comps), dtype=self.dtype)
+        _lib.check(_lib.lib().fvdbm_get_field(self._handle(), field, _lib.ptr(out)), "fvdbm_get_field")
+        return out
+
+    def _write(self, kind, name, value):
+        if (kind, name) not in _SETTABLE:
+            raise ValueError(f"{kind}.{name} is derived from the pdf and cannot be set")
+        field, comps = _FIELDS[(kind, name)]
+        self._upload(field, value, comps)
+
+    # stepping --------------------------------------------------------------
+    def step(self, nsteps=1):
+        """One FVDBM step (or ``nsteps``): macros -> equilibrium -> node pdfs
+        -> face fluxes -> cell update (src/environment.py:55-65). Returns self."""
+        _lib.check(_lib.lib().fvdbm_step(self._handle(), int(nsteps)), "fvdbm_step")
+        return self
+
+    def step_residual(self, nsteps=1):
+        """Advance ``nsteps`` and return R = ||u^{n+1}-u^n|| / ||u^{n+1}|| of
+        the last step (over all cells, FP64 accumulation)."""
+        r = ct.c_double()
+        _lib.check(_lib.lib().fvdbm_step_residual(self._handle(), int(nsteps), ct.byref(r)),
+                   "fvdbm_step_residual")
+        return float(r.value)
+
+    def run_to_steady(self, tol=1e-6, check_every=100, max_steps=1_000_000):
+        """Step until the residual of a checked step is below ``tol``.
+        Returns (steps taken, residual history)."""
+        hist = []
+        steps = 0
+        while steps < max_steps:
+            n = min(check_every, max_steps - steps)
+            r = self.step_residual(n)
+            steps += n
+            hist.append((steps, r))
+            if not np.isfinite(r) or r < tol:
+                break
+        return steps, hist
+
+    def time_steps(self, nsteps):
+        """Run ``nsteps`` between HIP events; returns elapsed milliseconds."""
+        ms = ct.c_double()
+        _lib.check(_lib.lib().fvdbm_step_timed(self._handle(), int(nsteps), ct.byref(ms)),
+                   "fvdbm_step_timed")
+        return float(ms.value)
+
+    def synchronize(self):
+        _lib.check(_lib.lib().fvdbm_synchronize(self._handle()), "fvdbm_synchronize")
+
+    def info(self):
+        inf = _lib.Info()
+        _lib.check(_lib.lib().fvdbm_info_get(self._handle(), ct.byref(inf)), "fvdbm_info_get")
+        return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+    def permutation(self):
+        out = np.empty(self.cells.size, np.int32)
+        _lib.check(_lib.lib().fvdbm_get_permutation(self._handle(), _lib.ptr(out)),
+                   "fvdbm_get_permutation")
+        return out
+
+    def __repr__(self):
+        return f"Environment(cells={self.cells}, faces={self.faces}, nodes={self.nodes})"

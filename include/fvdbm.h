@@ -1,0 +1,149 @@
+/*
+ * fvdbm.h — C ABI of the MI355X FVDBM solver library (libfvdbm.so).
+ *
+ * The reference (MADALERAU/FVDBM-JAX) has no FFI: its boundary is the Python
+ * object API. Each entry point below replaces one piece of that API and is
+ * bound from Python with ctypes (fvdbm-jax_amd/_lib.py):
+ *
+ *   fvdbm_create       Environment(cells, faces, nodes) + Environment.init()
+ *                      (src/environment.py:15-18, 38-42; Cells/Faces/Nodes.init
+ *                      src/containers.py:64-69, 157-163, 309-315) — uploads the
+ *                      static arrays Mesher.to_env produced (src/mesher.py:610-693)
+ *   fvdbm_init_uniform Container.__init__ pdf = calc_eq(1, 0) (src/containers.py:29)
+ *   fvdbm_set_field    assignment to a dynamic container attribute (pytree child)
+ *   fvdbm_get_field    reading cells.pdf / rho / vel / pdf_eq, faces.pdf,
+ *                      nodes.pdf / rho / vel (src/containers.py:53-62,146-155,299-307)
+ *   fvdbm_step         env = env.step()  (src/environment.py:55-65), n times
+ *   fvdbm_destroy      end of the Environment's lifetime
+ *
+ * Conventions: every entry point returns 0 on success and a non-zero
+ * FVDBM_ERR_* code otherwise; fvdbm_last_error() gives a message for the
+ * calling thread. A handle owns its device buffers and one HIP stream; it is
+ * not thread-safe, separate handles are independent. Arrays are host
+ * pointers in the reference's row-major layout and in the handle's precision
+ * (float for FVDBM_FP32, double for FVDBM_FP64) unless stated otherwise.
+ */
+#ifndef FVDBM_H
+#define FVDBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVDBM_ABI_VERSION 1
+
+enum {
+  FVDBM_OK = 0,
+  FVDBM_ERR_INVALID = 1, /* bad argument or mesh (ValueError in Python) */
+  FVDBM_ERR_HIP = 2,     /* HIP runtime error */
+  FVDBM_ERR_ALLOC = 3,   /* device or host allocation failed */
+  FVDBM_ERR_STATE = 4,   /* call not valid in the handle's state */
+};
+
+enum { FVDBM_FP32 = 0, FVDBM_FP64 = 1 };
+enum { FVDBM_UPWIND = 0, FVDBM_LAX_WENDROFF = 1 };
+
+/* Observable fields, each in reference order and layout. */
+enum {
+  FVDBM_CELL_PDF = 0,    /* [C][9]  cells.pdf (current) */
+  FVDBM_CELL_RHO = 1,    /* [C]     cells.rho    } moments of the pdf at the */
+  FVDBM_CELL_VEL = 2,    /* [C][2]  cells.vel    } start of the last step, as */
+  FVDBM_CELL_PDF_EQ = 3, /* [C][9]  cells.pdf_eq } the reference leaves them */
+  FVDBM_FACE_FLUX = 4,   /* [F][9]  faces.pdf: flux of the last step */
+  FVDBM_NODE_PDF = 5,    /* [N][9]  nodes.pdf */
+  FVDBM_NODE_RHO = 6,    /* [N]     nodes.rho */
+  FVDBM_NODE_VEL = 7,    /* [N][2]  nodes.vel */
+};
+
+/* Static mesh, exactly the arrays Mesher.to_env hands to the containers
+ * (src/mesher.py:632-691) plus the boundary-condition arrays that
+ * set_vel_node / set_rho_node fill (src/mesher.py:697-742). Float arrays are
+ * in the handle's precision. */
+typedef struct fvdbm_mesh_desc {
+  int32_t n_cells, n_faces, n_nodes, k_max;
+  const int32_t *cell_faces; /* [C][3] cells.face_indices */
+  const int32_t *cell_signs; /* [C][3] cells.face_normals (+-1) */
+  const int32_t *face_cells; /* [F][2] faces.stencil_cells_index (-1 = ghost) */
+  const void *face_dists;    /* [F][2] faces.stencil_dists */
+  const int32_t *face_nodes; /* [F][2] faces.nodes_index */
+  const void *face_n;        /* [F][2] faces.n */
+  const void *face_L;        /* [F]    faces.L */
+  const int32_t *node_cells; /* [N][K] nodes.cells_index (-1 padded) */
+  const void *node_dists;    /* [N][K] nodes.cell_dists (-1 padded) */
+  const int32_t *node_type;  /* [N]    nodes.type: 0 none, 1 velocity, 2 density */
+  const void *node_rho;      /* [N]    nodes.rho  (density-node values) */
+  const void *node_vel;      /* [N][2] nodes.vel  (velocity-node values) */
+} fvdbm_mesh_desc;
+
+/* D2Q9(tau, delta_t) (src/dynamics.py:65-68) and layout/launch options. */
+typedef struct fvdbm_params {
+  double tau;
+  double delta_t;
+  int32_t precision;  /* FVDBM_FP32 | FVDBM_FP64 */
+  int32_t scheme;     /* FVDBM_UPWIND | FVDBM_LAX_WENDROFF (Faces.flux_scheme) */
+  int32_t reorder;    /* 1: reverse Cuthill-McKee cell order (boundary cells first) */
+  int32_t variant;    /* step kernel variant, 0 = default (see fvdbm_variant_count) */
+  int32_t block;      /* threads per block (64..1024, multiple of 64), 0 = default */
+  int32_t graph_steps;/* steps per captured hipGraph, 0 = default, <0 = no graphs */
+} fvdbm_params;
+
+typedef struct fvdbm_info {
+  int32_t n_cells, n_faces, n_nodes, k_max;
+  int32_t n_cells_padded;
+  int32_t n_boundary_cells; /* cells with a ghost face (stored first) */
+  int32_t n_boundary_faces;
+  int32_t n_active_nodes;   /* nodes of type 1 or 2 */
+  int32_t bandwidth;        /* max |i - j| over neighbouring cells, device order */
+  int32_t precision, scheme, variant, block, graph_steps;
+  int64_t steps_done;
+  int64_t device_bytes;     /* device memory owned by the handle */
+} fvdbm_info;
+
+const char *fvdbm_last_error(void);
+int fvdbm_abi_version(void);
+/* Number of HIP devices visible (0 without a GPU; never an error). */
+int fvdbm_device_count(int32_t *count);
+/* Number of step-kernel variants the autotuner may choose from. */
+int fvdbm_variant_count(int32_t *count);
+
+int fvdbm_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *params,
+                 void **handle);
+int fvdbm_destroy(void *handle);
+int fvdbm_info_get(void *handle, fvdbm_info *out);
+
+/* Cell and node pdfs <- W; moments, pdf_eq zeroed; flux <- W
+ * (the state right after Environment.init()). */
+int fvdbm_init_uniform(void *handle);
+/* Write a field (FVDBM_CELL_PDF, FVDBM_NODE_PDF, FVDBM_NODE_RHO,
+ * FVDBM_NODE_VEL) from host memory in reference order. */
+int fvdbm_set_field(void *handle, int32_t field, const void *src);
+/* Read any FVDBM_* field into host memory in reference order. */
+int fvdbm_get_field(void *handle, int32_t field, void *dst);
+
+/* Advance nsteps steps (asynchronous on the handle's stream). */
+int fvdbm_step(void *handle, int64_t nsteps);
+/* Advance nsteps steps and return the residual of the last one,
+ * R = ||u^{n+1} - u^n||_2 / ||u^{n+1}||_2 over all cells (synchronous). */
+int fvdbm_step_residual(void *handle, int64_t nsteps, double *residual);
+/* Advance nsteps steps between two HIP events on the handle's stream and
+ * return the elapsed milliseconds (synchronous). */
+int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms);
+int fvdbm_synchronize(void *handle);
+
+/* Device cell order: new_to_old[i] = reference cell id of device cell i. */
+int fvdbm_get_permutation(void *handle, int32_t *new_to_old);
+
+/* Host-only (no device needed): the cell order fvdbm_create would use for
+ * this mesh with `reorder` (0 identity, 1 RCM with ghost-face cells first,
+ * 2 RCM), and stats = {boundary cells, boundary faces, active nodes,
+ * bandwidth}. Returns FVDBM_ERR_INVALID for an inconsistent mesh. */
+int fvdbm_layout_order(const fvdbm_mesh_desc *mesh, int32_t precision, int32_t reorder,
+                       int32_t *new_to_old, int32_t *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVDBM_H */

@@ -1,0 +1,84 @@
+"""Benchmark and test cases built through the product's Mesher, plus the
+to_env arrays in the dict layout the oracle takes."""
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+
+fv = importlib.import_module("fvdbm-jax_amd")
+meshgen = fv.meshgen
+
+LDC = dict(tau=0.6, dt=0.01, u_lid=0.1, dim_multiplier=1)
+
+
+def build_mesher(raw):
+    m = fv.Mesher.from_raw(raw)
+    m.enforce_ccw()
+    m.calc_mesh_properties()
+    return m
+
+
+def ldc_case(n=32, tau=LDC["tau"], dt=LDC["dt"], u_lid=LDC["u_lid"], scheme="upwind"):
+    """The 32x32 lid-driven cavity: lid nodes u=(u_lid,0), all other boundary
+    nodes (lid corners included) u=0, interior nodes type 0."""
+    m = build_mesher(meshgen.ldc_structured(n))
+    dyn = fv.D2Q9(tau=tau, delta_t=dt)
+    cells, faces, nodes = m.to_env(dyn, scheme, dim_multiplier=1)
+    nodes = m.set_vel_node(nodes, meshgen.LID, np.array([u_lid, 0.0]))
+    nodes = m.set_vel_node(nodes, meshgen.WALL, np.array([0.0, 0.0]))
+    return m, dyn, cells, faces, nodes
+
+
+def channel_case(raw, tau, dt, u_max, dim_multiplier, rho_out=1.0, scheme="upwind"):
+    """Velocity inlet (parabolic, per node), density outlet, no-slip walls and
+    obstacles. Walls are set last so channel corners are walls."""
+    m = build_mesher(raw)
+    dyn = fv.D2Q9(tau=tau, delta_t=dt)
+    cells, faces, nodes = m.to_env(dyn, scheme, dim_multiplier=dim_multiplier)
+    H = raw.meta.get("height", 0.41)
+    nodes = m.set_vel_profile(nodes, meshgen.INLET,
+                              lambda p: meshgen.parabolic_inlet(p[:, 1], H, u_max))
+    nodes = m.set_rho_node(nodes, meshgen.OUTLET, rho_out)
+    nodes = m.set_vel_node(nodes, meshgen.WALL, np.array([0.0, 0.0]))
+    if np.any(m.point_markers == meshgen.OBSTACLE):
+        nodes = m.set_vel_node(nodes, meshgen.OBSTACLE, np.array([0.0, 0.0]))
+    return m, dyn, cells, faces, nodes
+
+
+def oracle_env(cells, faces, nodes, precision="f32"):
+    """The to_env arrays as the oracle takes them."""
+    if precision == "f32":
+        return dict(cell_faces=cells.face_indices, cell_signs=cells.face_normals,
+                    face_cells=faces.stencil_cells_index, face_dists=faces.stencil_dists,
+                    face_nodes=faces.nodes_index, face_n=faces.n,
+                    face_L=np.asarray(faces.L).reshape(-1), node_cells=nodes.cells_index,
+                    node_dists=nodes.cell_dists)
+    return dict(cell_faces=cells.face_indices, cell_signs=cells.face_normals,
+                face_cells=faces.stencil_cells_index, face_dists=faces.stencil_dists_f64,
+                face_nodes=faces.nodes_index, face_n=faces.n_f64,
+                face_L=np.asarray(faces.L_f64).reshape(-1), node_cells=nodes.cells_index,
+                node_dists=nodes.cell_dists_f64)
+
+
+def oracle_state(env, nodes, precision="f32"):
+    from oracle import oracle
+    if precision == "f32":
+        return oracle.initial_state(env, nodes.type, nodes.__dict__["_host_rho"],
+                                    nodes.__dict__["_host_vel"], "f32")
+    return oracle.initial_state(env, nodes.type, nodes.rho_f64, nodes.vel_f64, "f64")
+
+
+def mean_area(mesher):
+    p = mesher.points[mesher.cells]
+    a = 0.5 * np.abs((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                     - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    return float(a.mean())
+
+
+def stable_dt(mesher, faces_L_scaled, tau, cfl=0.5):
+    """Explicit-Euler bound for the reference update without 1/A:
+    dt * (1/tau + max_cell sum_faces L' * sqrt(2)) <= cfl."""
+    L = np.asarray(faces_L_scaled, np.float64).reshape(-1)
+    per_cell = L[mesher.cell_face_indices].sum(axis=1) * np.sqrt(2.0)
+    return cfl / (1.0 / tau + per_cell.max())

@@ -1,0 +1,166 @@
+"""GPU parity: the HIP step (through the C ABI) against the CPU oracle.
+
+Integer work is bit-exact by construction and the floating-point step keeps
+the oracle's operation order with no contraction, so the bar here is bitwise
+equality (np.array_equal) of every observable array, in FP32 and in FP64,
+for both flux schemes, every cell order, kernel variant and block size.
+"""
+import numpy as np
+import pytest
+
+from cases import channel_case, fv, ldc_case, mean_area, meshgen, oracle_env, oracle_state, stable_dt
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [("cells", "pdf", "cell_pdf"), ("cells", "rho", "cell_rho"), ("cells", "vel", "cell_vel"),
+          ("cells", "pdf_eq", "cell_feq"), ("faces", "pdf", "face_pdf"),
+          ("nodes", "pdf", "node_pdf"), ("nodes", "rho", "node_rho"), ("nodes", "vel", "node_vel")]
+
+
+def _gpu_env(cells, faces, nodes, precision="fp32", **opts):
+    env = fv.Environment(cells, faces, nodes, precision=precision, **opts)
+    env.init()
+    return env
+
+
+def _assert_fields_equal(env, st, fields=FIELDS, what=""):
+    for kind, name, key in fields:
+        got = getattr(getattr(env, kind), name)
+        exp = np.asarray(st[key]).reshape(got.shape)
+        if not np.array_equal(got, exp):
+            bad = np.argwhere(got != exp)
+            d = np.max(np.abs(got.astype(np.float64) - exp))
+            raise AssertionError(f"{what} {kind}.{name}: {len(bad)} mismatches, max |diff| {d:.3e}, "
+                                 f"first at {bad[0].tolist()}")
+
+
+def test_initial_observables(gpu):
+    m, dyn, cells, faces, nodes = ldc_case()
+    env = _gpu_env(cells, faces, nodes)
+    W = np.float32(dyn.W)
+    assert np.array_equal(env.cells.pdf, np.repeat(W[None], 2048, 0))
+    assert not env.cells.rho.any() and not env.cells.vel.any() and not env.cells.pdf_eq.any()
+    assert np.array_equal(env.faces.pdf, np.repeat(W[None], 3136, 0))
+    assert np.array_equal(env.nodes.pdf, np.repeat(W[None], 1089, 0))
+    assert np.array_equal(env.nodes.vel, nodes.__dict__["_host_vel"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_cavity_matches_oracle(gpu, precision):
+    """32x32 cavity, 1000 steps (checked after 1, 10, 100 and 1000)."""
+    m, dyn, cells, faces, nodes = ldc_case()
+    p = "f32" if precision == "fp32" else "f64"
+    oenv = oracle_env(cells, faces, nodes, p)
+    st = oracle_state(oenv, nodes, p)
+    env = _gpu_env(cells, faces, nodes, precision)
+    done = 0
+    for target in (1, 10, 100, 1000):
+        oracle.run(oenv, st, dyn.tau, dyn.delta_t, target - done, p)
+        env = env.step(target - done)
+        done = target
+        _assert_fields_equal(env, st, FIELDS if target in (1, 1000) else FIELDS[:1], f"step {target}")
+
+
+def test_cavity_lax_wendroff(gpu):
+    m, dyn, cells, faces, nodes = ldc_case(scheme="lax_wendroff")
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 300, "f32", scheme="lax_wendroff")
+    env = _gpu_env(cells, faces, nodes)
+    env.step(300)
+    _assert_fields_equal(env, st)
+
+
+def _channel(target_cells=20_000, seed=0, scheme="upwind"):
+    raw = meshgen.cylinder_channel(target_cells, seed=seed)
+    from cases import build_mesher
+    mm = build_mesher(raw)
+    mult = 1.0 / mean_area(mm)
+    m, dyn, cells, faces, nodes = channel_case(raw, tau=0.55, dt=1.0, u_max=0.1,
+                                               dim_multiplier=mult, scheme=scheme)
+    dyn.delta_t = float(np.float32(stable_dt(m, faces.L, dyn.tau)))
+    return m, dyn, cells, faces, nodes
+
+
+@pytest.mark.parametrize("scheme", ["upwind", "lax_wendroff"])
+def test_channel_matches_oracle(gpu, scheme):
+    """Cylinder channel: parabolic velocity inlet (per-node values), density
+    outlet (type-2 nodes), no-slip walls and cylinder."""
+    m, dyn, cells, faces, nodes = _channel(scheme=scheme)
+    assert (nodes.type == 2).any() and (nodes.type == 1).any()
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 200, "f32", scheme=scheme)
+    env = _gpu_env(cells, faces, nodes)
+    env.step(200)
+    _assert_fields_equal(env, st)
+
+
+@pytest.mark.parametrize("opts", [
+    dict(reorder=0), dict(reorder=2), dict(variant=1), dict(variant=2), dict(variant=3),
+    dict(block=64), dict(block=512), dict(block=1024), dict(graph_steps=-1), dict(graph_steps=6),
+])
+def test_layout_and_variants_bitwise_equal(gpu, opts):
+    """Cell order (n1), kernel variant / block size (autotuner candidates) and
+    graph capture change nothing in the results."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=1)
+    base = _gpu_env(cells, faces, nodes)
+    base.step(77)
+    ref = base.cells.pdf
+    ref_flux = base.faces.pdf
+    base.close()
+    env = _gpu_env(cells, faces, nodes, **opts)
+    env.step(77)
+    assert np.array_equal(env.cells.pdf, ref)
+    assert np.array_equal(env.faces.pdf, ref_flux)
+
+
+def test_set_state_and_residual(gpu):
+    m, dyn, cells, faces, nodes = ldc_case()
+    rng = np.random.default_rng(0)
+    f0 = (np.float32(dyn.W) * (1 + 0.05 * rng.standard_normal((2048, 9)))).astype(np.float32)
+    env = _gpu_env(cells, faces, nodes)
+    env.cells.pdf = f0
+    assert np.array_equal(env.cells.pdf, f0)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    st["cell_pdf"] = f0.copy()
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 9, "f32")
+    before = st["cell_pdf"].copy()
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 1, "f32")
+    r = env.step_residual(10)
+    _assert_fields_equal(env, st)
+    _, u0 = oracle.calc_macro(before)
+    _, u1 = oracle.calc_macro(st["cell_pdf"])
+    d = (u1 - u0).astype(np.float32)
+    num = np.sum((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(np.float32), dtype=np.float64)
+    den = np.sum((u1[:, 0] * u1[:, 0] + u1[:, 1] * u1[:, 1]).astype(np.float32), dtype=np.float64)
+    assert abs(r - np.sqrt(num) / np.sqrt(den)) <= 1e-9 * r
+
+
+def test_moments_describe_start_of_last_step(gpu):
+    """cells.rho/vel/pdf_eq after step() are those of the pdf the step read
+    (src/environment.py:60-64)."""
+    m, dyn, cells, faces, nodes = ldc_case()
+    env = _gpu_env(cells, faces, nodes)
+    env.step(5)
+    prev = env.cells.pdf
+    env.step()
+    rho, vel = oracle.calc_macro(prev)
+    assert np.array_equal(env.cells.rho.reshape(-1), rho)
+    assert np.array_equal(env.cells.vel, vel)
+    assert np.array_equal(env.cells.pdf_eq, oracle.calc_eq(rho, vel))
+
+
+def test_every_cell_written_and_deterministic(gpu):
+    """Two runs agree bitwise and every cell holds a finite pdf."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=2)
+    outs = []
+    for _ in range(2):
+        env = _gpu_env(cells, faces, nodes)
+        env.step(50)
+        outs.append(env.cells.pdf)
+        env.close()
+    assert np.isfinite(outs[0]).all()
+    assert np.array_equal(outs[0], outs[1])
