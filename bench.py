@@ -1,0 +1,159 @@
+"""Benchmark: MLUPS of the FVDBM step on the 1M-cell cylinder channel.
+
+Workload: channel [0,2.2]x[0,0.41] with a cylinder r=0.05 at (0.2,0.2),
+Delaunay mesh of seeded jittered points (~1.0M cells), parabolic velocity
+inlet u_max=0.1, density outlet rho=1, no-slip walls and cylinder, D2Q9,
+tau=0.55, first-order upwind, FP32. The update has no 1/A term (as in the
+reference), so dim_multiplier = 1/mean(cell area) and dt is the explicit
+stability bound at CFL 0.5; both are reported.
+
+One step = one fused kernel launch over all cells (k_step). The timed region
+is K steps between two HIP events on the solver's stream (launched as
+captured hipGraphs), with the inputs resident in HBM. value = cells * K /
+elapsed / 1e6 (MLUPS). roofline.achieved = 108 B/LUP * cells / (elapsed/K).
+cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
+of the same mesh, on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_LUP_F32 = 108  # pdf 36 read + 36 write, 3 neighbour ids 12, 3 (L n) pairs 24
+BYTES_PER_LUP_F64 = 204
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_case(cells, seed, scheme="upwind"):
+    fv = importlib.import_module("fvdbm-jax_amd")
+    mg = fv.meshgen
+    t0 = time.time()
+    raw = mg.cylinder_channel(cells, seed=seed)
+    m = fv.Mesher.from_raw(raw)
+    m.enforce_ccw()
+    m.calc_mesh_properties()
+    p = m.points[m.cells]
+    area = 0.5 * np.abs((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                        - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    mult = float(1.0 / area.mean())
+    tau = 0.55
+    dyn = fv.D2Q9(tau=tau, delta_t=1.0)
+    c, f, n = m.to_env(dyn, scheme, dim_multiplier=mult)
+    L = np.asarray(f.L, np.float64).reshape(-1)
+    per_cell = L[m.cell_face_indices].sum(axis=1) * np.sqrt(2.0)
+    dyn.delta_t = float(np.float32(0.5 / (1.0 / tau + per_cell.max())))
+    n = m.set_vel_profile(n, mg.INLET, lambda q: mg.parabolic_inlet(q[:, 1], 0.41, 0.1))
+    n = m.set_rho_node(n, mg.OUTLET, 1.0)
+    n = m.set_vel_node(n, mg.WALL, np.array([0.0, 0.0]))
+    n = m.set_vel_node(n, mg.OBSTACLE, np.array([0.0, 0.0]))
+    log(f"mesh: {len(m.cells)} cells, {len(m.faces)} faces, {len(m.points)} nodes "
+        f"({time.time() - t0:.1f}s)")
+    return fv, m, dyn, c, f, n, dict(dim_multiplier=mult, dt=dyn.delta_t, tau=tau)
+
+
+def cpu_baseline(m, dyn, c, f, n, budget_s=12.0):
+    """The oracle (scalar C, 1 thread) on a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle
+    env = dict(cell_faces=c.face_indices, cell_signs=c.face_normals,
+               face_cells=f.stencil_cells_index, face_dists=f.stencil_dists,
+               face_nodes=f.nodes_index, face_n=f.n, face_L=np.asarray(f.L).reshape(-1),
+               node_cells=n.cells_index, node_dists=n.cell_dists)
+    st = oracle.initial_state(env, n.type, n.__dict__["_host_rho"], n.__dict__["_host_vel"], "f32")
+    t0 = time.perf_counter()
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f32")
+    one = time.perf_counter() - t0
+    steps = max(1, min(200, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.run(env, st, dyn.tau, dyn.delta_t, steps, "f32")
+    el = time.perf_counter() - t0
+    C = c.face_indices.shape[0]
+    return dict(value=round(C * steps / el / 1e6, 3), unit="MLUPS", cores=1, kind="port",
+                sample=f"{steps} steps of the same {C}-cell mesh, FP32, oracle C restatement "
+                       f"(oracle/fvdbm_oracle.c, gcc -O2), 1 thread, {el:.1f}s")
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build targets a single MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed)
+    C = len(m.cells)
+    env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block)
+    t0 = time.time()
+    env.init()
+    info = env.info()
+    log(f"upload {time.time() - t0:.1f}s; bandwidth {info['bandwidth']}, boundary cells "
+        f"{info['n_boundary_cells']}, device bytes {info['device_bytes'] / 1e6:.1f} MB")
+    env.step(args.warmup)
+    env.synchronize()
+    ms = env.time_steps(args.steps)
+    r = env.step_residual(1)
+    if not np.isfinite(r):
+        raise SystemExit("non-finite residual: the run diverged")
+    mlups = C * args.steps / (ms / 1e3) / 1e6
+    per_launch_s = ms / 1e3 / args.steps
+    bpl = BYTES_PER_LUP_F32 if args.precision == "fp32" else BYTES_PER_LUP_F64
+    achieved = bpl * C / per_launch_s / 1e9
+    workload = f"cylinder_channel_{C // 1000}k" if C < 999_500 else f"cylinder_channel_{round(C / 1e6)}M"
+    tr = load_traffic("cylinder_channel_1M") if abs(C - 1_000_000) < 50_000 else None
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=(tr["bytes_per_launch"] if tr else None))
+    cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
+    out = {
+        "metric": "MLUPS at 1M cells (%HBM roofline)",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded jittered-Delaunay mesh, uniform equilibrium start)",
+        "config": {"workload": workload, "cells": C, "faces": len(m.faces), "nodes": len(m.points),
+                   "scheme": "upwind", "tau": case["tau"], "dt": case["dt"],
+                   "dim_multiplier": case["dim_multiplier"], "seed": args.seed,
+                   "kernel": "k_step (fused cell step)", "bytes_per_lup": bpl,
+                   "residual_after": r},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
