@@ -39,11 +39,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_case(cells, seed, scheme="upwind"):
+def build_case(cells, seed, scheme="upwind", cache_dir=None):
     fv = importlib.import_module("fvdbm-jax_amd")
     mg = fv.meshgen
     t0 = time.time()
-    raw = mg.cylinder_channel(cells, seed=seed)
+    path = os.path.join(cache_dir, f"cylinder_{cells}_{seed}.npz") if cache_dir else None
+    if path and os.path.exists(path):
+        z = np.load(path)
+        raw = mg.RawMesh(z["points"], z["cells"], z["faces"], z["point_markers"])
+    else:
+        raw = mg.cylinder_channel(cells, seed=seed)
+        if path:
+            os.makedirs(cache_dir, exist_ok=True)
+            np.savez(path, points=raw.points, cells=raw.cells, faces=raw.faces,
+                     point_markers=raw.point_markers)
     m = fv.Mesher.from_raw(raw)
     m.enforce_ccw()
     m.calc_mesh_properties()
@@ -106,16 +115,29 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--reorder", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--mesh-cache", default=None,
+                    help="directory to keep the generated mesh between runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
-    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed)
+    copy = None
+    if args.copy_probe:
+        _lib = importlib.import_module("fvdbm-jax_amd._lib")
+        nbytes = 256 << 20
+        cms = _lib.copy_probe(nbytes, 20)
+        copy = dict(bytes=nbytes, ms=round(cms, 5), GBps=round(2 * nbytes / (cms / 1e3) / 1e9, 1))
+        log(f"copy probe: 256 MiB in {cms:.4f} ms -> {copy['GBps']} GB/s (read + write)")
+    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache)
     C = len(m.cells)
-    env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block)
+    env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block,
+                         reorder=args.reorder)
     t0 = time.time()
     env.init()
     info = env.info()
@@ -151,6 +173,7 @@ def main():
                    "residual_after": r},
         "roofline": roof,
         "cpu_baseline": cpu,
+        **({"copy_probe": copy} if copy else {}),
     }
     print(json.dumps(out), flush=True)
 

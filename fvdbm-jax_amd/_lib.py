@@ -31,7 +31,8 @@ class MeshDesc(ct.Structure):
                 ("face_nodes", ct.c_void_p), ("face_n", ct.c_void_p),
                 ("face_L", ct.c_void_p), ("node_cells", ct.c_void_p),
                 ("node_dists", ct.c_void_p), ("node_type", ct.c_void_p),
-                ("node_rho", ct.c_void_p), ("node_vel", ct.c_void_p)]
+                ("node_rho", ct.c_void_p), ("node_vel", ct.c_void_p),
+                ("cell_xy", ct.c_void_p)]
 
 
 class Params(ct.Structure):
@@ -66,6 +67,7 @@ _SIGNATURES = {
     "fvdbm_step_timed": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
     "fvdbm_synchronize": (ct.c_int, [ct.c_void_p]),
     "fvdbm_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "fvdbm_copy_probe": (ct.c_int, [ct.c_int64, ct.c_int32, ct.POINTER(ct.c_double)]),
     "fvdbm_layout_order": (ct.c_int, [ct.POINTER(MeshDesc), ct.c_int32, ct.c_int32,
                                       ct.c_void_p, ct.c_void_p]),
 }
@@ -135,7 +137,8 @@ def ptr(a):
 def mesh_desc(arrays, dtype):
     """MeshDesc over a dict of contiguous arrays (kept alive by the caller).
     Keys: cell_faces, cell_signs, face_cells, face_dists, face_nodes, face_n,
-    face_L, node_cells, node_dists, node_type, node_rho, node_vel."""
+    face_L, node_cells, node_dists, node_type, node_rho, node_vel, and the
+    optional cell_xy (centroids, for the Hilbert cell order)."""
     a = arrays
     for k in ("cell_faces", "cell_signs", "face_cells", "face_nodes", "node_cells", "node_type"):
         a[k] = np.ascontiguousarray(a[k], dtype=np.int32)
@@ -145,7 +148,17 @@ def mesh_desc(arrays, dtype):
     F = a["face_nodes"].shape[0]
     N = a["node_type"].reshape(-1).shape[0]
     K = a["node_cells"].shape[1] if a["node_cells"].ndim == 2 else 0
+    xy = a.get("cell_xy")
+    if xy is not None:
+        a["cell_xy"] = xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(C, 2)
     return MeshDesc(C, F, N, K, *[ptr(a[k]) for k in (
         "cell_faces", "cell_signs", "face_cells", "face_dists", "face_nodes",
         "face_n", "face_L", "node_cells", "node_dists", "node_type",
-        "node_rho", "node_vel")])
+        "node_rho", "node_vel")], ptr(xy))
+
+
+def copy_probe(nbytes, reps=20):
+    """Milliseconds per device copy of ``nbytes`` (k_copy_probe)."""
+    ms = ct.c_double()
+    check(lib().fvdbm_copy_probe(int(nbytes), int(reps), ct.byref(ms)), "fvdbm_copy_probe")
+    return float(ms.value)

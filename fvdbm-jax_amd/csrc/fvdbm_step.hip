@@ -65,8 +65,9 @@ __host__ __device__ constexpr int ky(int q) {
 template <typename T>
 struct Lattice {
   T W[Q];
-  T C2, C4x2, C2x2;  // C*C, 2*(C*C)*(C*C), 2*C*C with C = 1/sqrt(3)
-  T invtau, dt;      // 1/tau rounded from double, delta_t rounded
+  T C2, C4x2, C2x2;     // C*C, 2*(C*C)*(C*C), 2*C*C with C = 1/sqrt(3)
+  T rC2, rC4x2, rC2x2;  // their rounded reciprocals (see div_const)
+  T invtau, dt;         // 1/tau rounded from double, delta_t rounded
 };
 
 template <typename T>
@@ -80,6 +81,9 @@ static Lattice<T> make_lattice(double tau, double dt) {
   T c4 = k.C2 * k.C2;
   k.C4x2 = T(2) * c4;
   k.C2x2 = T(2) * k.C2;
+  k.rC2 = T(1) / k.C2;
+  k.rC4x2 = T(1) / k.C4x2;
+  k.rC2x2 = T(1) / k.C2x2;
   k.invtau = T(1.0 / tau);
   k.dt = T(dt);
   return k;
@@ -110,18 +114,63 @@ __device__ __forceinline__ void macro(const T (&f)[Q], T &rho, T &ux, T &uy) {
 // B = (ksi.u)^2/(2C^4), D = (u.u)/(2C^2). ksi.u of opposite directions are
 // exact negatives and (-a)/b == -(a/b) in round-to-nearest, so only four A
 // and four B quotients are distinct; each bracket is still formed per q.
+// x / y for one of the three lattice constants y with r = RN(1/y).
+// FP32: q0 = RN(x r), rem = x - q0 y (exact with one FMA), q = RN(q0 + rem r).
+// An exhaustive CPU check over all 2^32 inputs shows q equals the IEEE
+// quotient x / y for every x whose biased exponent lies in [25, 251], for
+// each of the three constants; the kernels use it when every dividend of an
+// equilibrium lies in [40, 251] (remainders stay normal) and the IEEE
+// division otherwise (zero, tiny, huge, inf, nan).
+__device__ __forceinline__ bool fast_div_ok(float x) {
+  return ((__float_as_uint(x) >> 23) & 0xffu) - 40u <= 211u;
+}
+__device__ __forceinline__ float fast_div(float x, float y, float r) {
+  const float q0 = x * r;
+  const float rem = __builtin_fmaf(-q0, y, x);
+  return __builtin_fmaf(rem, r, q0);
+}
+
+template <typename T>
+struct EqTerms {
+  T D, A1, A2, A5, A6, B1, B2, B5, B6;
+};
+
+template <typename T>
+__device__ __forceinline__ EqTerms<T> eq_terms(const Lattice<T> &k, T ux, T uy, T uu, T c5,
+                                               T c6) {
+  return {uu / k.C2x2,        ux / k.C2,          uy / k.C2,
+          c5 / k.C2,          c6 / k.C2,          (ux * ux) / k.C4x2,
+          (uy * uy) / k.C4x2, (c5 * c5) / k.C4x2, (c6 * c6) / k.C4x2};
+}
+template <>
+__device__ __forceinline__ EqTerms<float> eq_terms<float>(const Lattice<float> &k, float ux,
+                                                          float uy, float uu, float c5, float c6) {
+  const float s1 = ux * ux, s2 = uy * uy, s5 = c5 * c5, s6 = c6 * c6;
+  const bool ok = fast_div_ok(uu) & fast_div_ok(ux) & fast_div_ok(uy) & fast_div_ok(c5) &
+                  fast_div_ok(c6) & fast_div_ok(s1) & fast_div_ok(s2) & fast_div_ok(s5) &
+                  fast_div_ok(s6);
+  if (ok) {
+    return {fast_div(uu, k.C2x2, k.rC2x2), fast_div(ux, k.C2, k.rC2),
+            fast_div(uy, k.C2, k.rC2),     fast_div(c5, k.C2, k.rC2),
+            fast_div(c6, k.C2, k.rC2),     fast_div(s1, k.C4x2, k.rC4x2),
+            fast_div(s2, k.C4x2, k.rC4x2), fast_div(s5, k.C4x2, k.rC4x2),
+            fast_div(s6, k.C4x2, k.rC4x2)};
+  }
+  return {uu / k.C2x2, ux / k.C2, uy / k.C2, c5 / k.C2, c6 / k.C2,
+          s1 / k.C4x2, s2 / k.C4x2, s5 / k.C4x2, s6 / k.C4x2};
+}
+
 template <typename T>
 __device__ __forceinline__ void equilibrium(const Lattice<T> &k, T rho, T ux, T uy,
                                             T (&feq)[Q]) {
   T uu = T(0);
   uu += ux * ux;
   uu += uy * uy;
-  const T D = uu / k.C2x2;
   const T c5 = ux + uy;          // q5; q7 = -c5
   const T c6 = (-ux) + uy;       // q6; q8 = -c6
-  const T A1 = ux / k.C2, A2 = uy / k.C2, A5 = c5 / k.C2, A6 = c6 / k.C2;
-  const T B1 = (ux * ux) / k.C4x2, B2 = (uy * uy) / k.C4x2;
-  const T B5 = (c5 * c5) / k.C4x2, B6 = (c6 * c6) / k.C4x2;
+  const EqTerms<T> t = eq_terms(k, ux, uy, uu, c5, c6);
+  const T D = t.D, A1 = t.A1, A2 = t.A2, A5 = t.A5, A6 = t.A6;
+  const T B1 = t.B1, B2 = t.B2, B5 = t.B5, B6 = t.B6;
   const T one = T(1);
   feq[0] = (k.W[0] * rho) * (((one + T(0)) + T(0)) - D);
   feq[1] = (k.W[1] * rho) * (((one + A1) + B1) - D);
@@ -134,14 +183,83 @@ __device__ __forceinline__ void equilibrium(const Lattice<T> &k, T rho, T ux, T 
   feq[8] = (k.W[8] * rho) * (((one - A6) + B6) - D);
 }
 
+// ----------------------------------------------------------------------------
+// raw buffer access: 32-bit per-lane byte offsets plus a uniform (SGPR)
+// offset; loads past num_records return 0 and stores past it are dropped
+// ----------------------------------------------------------------------------
+constexpr int32_t kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer descriptor
+
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const void *p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                            kRsrcWord3)) {}
+  template <typename T>
+  __device__ __forceinline__ T ld(uint32_t voff, uint32_t soff) const;
+  template <typename T>
+  __device__ __forceinline__ void st(T v, uint32_t voff, uint32_t soff) const;
+  __device__ __forceinline__ uint4 ld128(uint32_t voff, uint32_t soff) const {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+  }
+};
+template <>
+__device__ __forceinline__ float Buf::ld<float>(uint32_t voff, uint32_t soff) const {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double Buf::ld<double>(uint32_t voff, uint32_t soff) const {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return __builtin_bit_cast(double, v);
+}
+template <>
+__device__ __forceinline__ int32_t Buf::ld<int32_t>(uint32_t voff, uint32_t soff) const {
+  return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void Buf::st<float>(float v, uint32_t voff, uint32_t soff) const {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void Buf::st<double>(double v, uint32_t voff, uint32_t soff) const {
+  using v2u = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, voff, soff, 0);
+}
+
+// One face of one cell: neighbour code, face normal, sign * length.
+// FP32: one 16-byte record; FP64: two 16-byte records (code, pad, nx | ny, sL).
+template <typename T>
+struct FaceRec {
+  int32_t code;
+  T nx, ny, sL;
+};
+template <typename T>
+struct GeoLayout;
+template <>
+struct GeoLayout<float> {
+  static constexpr uint32_t bytes = 16;
+  __device__ static __forceinline__ FaceRec<float> load(const Buf &b, uint32_t vo, uint32_t so) {
+    const uint4 v = b.ld128(vo, so);
+    return {(int32_t)v.x, __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
+  }
+};
+template <>
+struct GeoLayout<double> {
+  static constexpr uint32_t bytes = 32;
+  __device__ static __forceinline__ FaceRec<double> load(const Buf &b, uint32_t vo, uint32_t so) {
+    const uint4 v0 = b.ld128(vo, so), v1 = b.ld128(vo + 16, so);
+    auto d = [](uint32_t lo, uint32_t hi) {
+      return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    };
+    return {(int32_t)v0.x, d(v0.z, v0.w), d(v1.x, v1.y), d(v1.z, v1.w)};
+  }
+};
+
 template <typename T>
 struct StepArgs {
-  const T *__restrict__ fsrc;  // [9][Cp]
-  T *__restrict__ fdst;        // [9][Cp]
-  const int32_t *__restrict__ nb;  // [3][Cp]
-  const T *__restrict__ nx;
-  const T *__restrict__ ny;
-  const T *__restrict__ sL;
+  const T *fsrc;    // [9][Cp]
+  T *fdst;          // [9][Cp]
+  const void *geo;  // [3][Cp] FaceRec records (GeoLayout<T>::bytes each)
   const T *__restrict__ lwa;
   const T *__restrict__ lwden2;
   const int32_t *__restrict__ bf_slot;   // [2][nbf] slots in the tile node table
@@ -149,6 +267,10 @@ struct StepArgs {
   const T *__restrict__ bf_ratio;        // [nbf]
   const int32_t *__restrict__ tile_node_off;  // [ntiles+1]
   const int32_t *__restrict__ tile_nodes;     // encoded nodes of each tile
+  const int32_t *__restrict__ tile_pair_off;  // [ntiles+1]
+  const int32_t *__restrict__ entry_pair_off; // [entries+1]
+  const int32_t *__restrict__ pair_cell;      // (node, cell) pairs
+  const T *__restrict__ pair_w;
   const int32_t *__restrict__ an_type;
   const int32_t *__restrict__ an_cells;  // [nan][K]
   const T *__restrict__ an_w;            // [nan][K]
@@ -159,14 +281,16 @@ struct StepArgs {
   const T *__restrict__ node_pdf;  // [9][N]
   double *__restrict__ partials;   // [ntiles][2] residual partial sums
   int C, Cp, N, K, nbf, nan;
-  int ntiles, xcd_remap, inline_nodes;
+  int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs;
   Lattice<T> k;
 };
 
+// The 9 pdfs of cell c from a q-major [9][Cp] buffer.
 template <typename T>
-__device__ __forceinline__ void load_cell(const T *__restrict__ f, int Cp, int c, T (&out)[Q]) {
+__device__ __forceinline__ void load_cell(const Buf &f, uint32_t plane, int c, T (&out)[Q]) {
+  const uint32_t vo = (uint32_t)c * sizeof(T);
 #pragma unroll
-  for (int q = 0; q < Q; ++q) out[q] = f[(size_t)q * Cp + c];
+  for (int q = 0; q < Q; ++q) out[q] = f.ld<T>(vo, q * plane);
 }
 
 // Nodes.calc_dirichlet_pdf for an active node (src/containers.py:343-390):
@@ -174,8 +298,8 @@ __device__ __forceinline__ void load_cell(const T *__restrict__ f, int Cp, int c
 // u <- IDW(cell u), rho fixed; pdf = calc_eq(rho, u) + IDW(f_c - feq_c).
 // Padding entries (-1) carry weight 0 and contribute +0, as x*0 does.
 template <typename T>
-__device__ __forceinline__ void active_node(const StepArgs<T> &a, int an, T (&out)[Q], T &rho_n,
-                                            T &ux_n, T &uy_n) {
+__device__ __forceinline__ void active_node(const StepArgs<T> &a, const Buf &fb, uint32_t plane,
+                                            int an, T (&out)[Q], T &rho_n, T &ux_n, T &uy_n) {
   const int type = a.an_type[an];
   const int32_t *cells = a.an_cells + (size_t)an * a.K;
   const T *w = a.an_w + (size_t)an * a.K;
@@ -191,7 +315,7 @@ __device__ __forceinline__ void active_node(const StepArgs<T> &a, int an, T (&ou
       T r = T(0), ux = T(0), uy = T(0);
       if (!(cj < 0 && wj == T(0))) {
         T fc[Q];
-        load_cell(a.fsrc, a.Cp, cj < 0 ? -2 - cj : cj, fc);
+        load_cell(fb, plane, cj < 0 ? -2 - cj : cj, fc);
         macro(fc, r, ux, uy);
       }
       if (type == 1) {
@@ -219,7 +343,7 @@ __device__ __forceinline__ void active_node(const StepArgs<T> &a, int an, T (&ou
       for (int q = 0; q < Q; ++q) acc[q] += T(0) * wj;
     } else {
       T fc[Q], eq[Q], r, ux, uy;
-      load_cell(a.fsrc, a.Cp, cj, fc);
+      load_cell(fb, plane, cj, fc);
       macro(fc, r, ux, uy);
       equilibrium(a.k, r, ux, uy, eq);
 #pragma unroll
@@ -282,15 +406,73 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   T *sh = reinterpret_cast<T *>(smem_raw);
   const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
   if (tile >= a.ntiles) return;  // uniform per block
+  const uint32_t plane = (uint32_t)a.Cp * sizeof(T);  // bytes between q planes
+  const Buf fin(a.fsrc, Q * plane);
   const int t0 = a.tile_node_off[tile];
   const int ntn = a.tile_node_off[tile + 1] - t0;
-  if (a.inline_nodes && ntn > 0) {  // uniform per block
-    for (int i = threadIdx.x; i < ntn; i += blockDim.x) {
+  const bool nodes_here = a.inline_nodes && ntn > 0;  // uniform per block
+  const int bs = blockDim.x;
+  if (nodes_here) {
+    // 1a: one thread per (node, cell) pair: that cell's weighted IDW
+    //     contributions [r w, ux w, uy w, (f_q - feq_q) w] into LDS
+    const int pb = a.tile_pair_off[tile], np = a.tile_pair_off[tile + 1] - pb;
+    T *con = sh + Q * a.max_tile_nodes;
+    for (int p = threadIdx.x; p < np; p += bs) {
+      const int cj = a.pair_cell[pb + p];
+      const T wj = a.pair_w[pb + p];
+      T fc[Q], r, ux, uy;
+      load_cell(fin, plane, cj < 0 ? -2 - cj : cj, fc);
+      macro(fc, r, ux, uy);
+      T *o = con + 12 * p;
+      o[0] = r * wj;
+      o[1] = ux * wj;
+      o[2] = uy * wj;
+      if (cj >= 0) {
+        T eq[Q];
+        equilibrium(a.k, r, ux, uy, eq);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) o[3 + q] = (fc[q] - eq[q]) * wj;
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) o[3 + q] = T(0) * wj;  // padding: neq is 0
+      }
+    }
+    __syncthreads();
+    // 1b: one thread per table entry: the IDW sums in nodes.cells_index
+    //     order (0-initialised, as the reference's jnp.sum), then the
+    //     Dirichlet pdf feq(rho, u) + IDW(neq) (src/containers.py:353-361)
+    for (int i = threadIdx.x; i < ntn; i += bs) {
       const int e = a.tile_nodes[t0 + i];
       T out[Q];
       if (e >= 0) {
-        T r, ux, uy;
-        active_node(a, e, out, r, ux, uy);
+        const int type = a.an_type[e];
+        const int p0 = a.entry_pair_off[t0 + i] - pb, p1 = a.entry_pair_off[t0 + i + 1] - pb;
+        T n0 = T(0), n1 = T(0), acc[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] = T(0);
+        for (int p = p0; p < p1; ++p) {
+          const T *o = con + 12 * p;
+          if (type == 1) {
+            n0 += o[0];
+          } else {
+            n0 += o[1];
+            n1 += o[2];
+          }
+#pragma unroll
+          for (int q = 0; q < Q; ++q) acc[q] += o[3 + q];
+        }
+        const T wsum = a.an_wsum[e];
+        T rho_n = a.an_rho[e], ux_n = a.an_ux[e], uy_n = a.an_uy[e];
+        if (type == 1) {
+          rho_n = n0 / wsum;
+        } else {
+          ux_n = n0 / wsum;
+          uy_n = n1 / wsum;
+        }
+        T eqn[Q];
+        equilibrium(a.k, rho_n, ux_n, uy_n, eqn);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) out[q] = eqn[q] + acc[q] / wsum;
       } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) out[q] = a.node_pdf[(size_t)q * a.N + (-1 - e)];
@@ -298,25 +480,50 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) sh[q * ntn + i] = out[q];
     }
-    __syncthreads();
   }
 
-  const int c = tile * blockDim.x + threadIdx.x;
+  // the tile's own pdfs: registers, and LDS ([9][blockDim]) so that
+  // neighbours inside the tile are read from LDS instead of L2/HBM
+  const int base = tile * bs;
+  const int c = base + threadIdx.x;
+  T *shf = sh + Q * a.max_tile_nodes + 12 * a.max_tile_pairs;
+  T f[Q];
+  if (c < a.C) {
+    load_cell(fin, plane, c, f);
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = T(0);
+  }
+  if (a.stage) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) shf[q * bs + threadIdx.x] = f[q];
+  }
+  if (nodes_here || a.stage) __syncthreads();
+
   T res_d = T(0), res_n = T(0);
   if (c < a.C) {
-    T f[Q], acc[Q];
-    load_cell(a.fsrc, a.Cp, c, f);
+    const Buf geo(a.geo, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
+    const uint32_t gplane = (uint32_t)a.Cp * GeoLayout<T>::bytes;
+    const uint32_t gvo = (uint32_t)c * GeoLayout<T>::bytes;
+    T acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = T(0);
 #pragma unroll
     for (int kf = 0; kf < 3; ++kf) {
-      const size_t at = (size_t)kf * a.Cp + c;
-      const int32_t code = a.nb[at];
-      const T nx = a.nx[at], ny = a.ny[at], sL = a.sL[at];
+      const FaceRec<T> fr = GeoLayout<T>::load(geo, gvo, kf * gplane);
+      const int32_t code = fr.code;
+      const T nx = fr.nx, ny = fr.ny, sL = fr.sL;
       const bool slot1 = (code & kSlot1Bit) != 0;
       T o[Q];
       if (code >= 0) {
-        load_cell(a.fsrc, a.Cp, code & kIndexMask, o);
+        const int nbr = code & kIndexMask;
+        const unsigned loc = (unsigned)(nbr - base);
+        if (a.stage && loc < (unsigned)bs) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) o[q] = shf[q * bs + loc];
+        } else {
+          load_cell(fin, plane, nbr, o);
+        }
       } else {
         // ghost (Faces.calc_ghost + utils.extrap_pdf):
         // fbar = mean(f_p, f_r); g = fbar + (fbar - f_self) * (d_ghost / d_known)
@@ -351,6 +558,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
           acc[q] += (up * phi) * sL;
         }
       } else {
+        const size_t at = (size_t)kf * a.Cp + c;
         const T la = a.lwa[at], den2 = a.lwden2[at];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -368,11 +576,13 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
     T rho, ux, uy, feq[Q];
     macro(f, rho, ux, uy);
     equilibrium(a.k, rho, ux, uy, feq);
+    const Buf fout(a.fdst, Q * plane);
+    const uint32_t vo = (uint32_t)c * sizeof(T);
     T fn[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       fn[q] = f[q] + a.k.dt * (a.k.invtau * (feq[q] - f[q]) - acc[q]);
-      a.fdst[(size_t)q * a.Cp + c] = fn[q];
+      fout.st<T>(fn[q], vo, q * plane);
     }
     if constexpr (RESID) {
       T r2, vx, vy;
@@ -392,8 +602,10 @@ __global__ void k_nodes(StepArgs<T> a, const int32_t *__restrict__ an_gid, T *__
                         T *__restrict__ nrho, T *__restrict__ nvel) {
   const int an = blockIdx.x * blockDim.x + threadIdx.x;
   if (an >= a.nan) return;
+  const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
+  const Buf fin(a.fsrc, Q * plane);
   T out[Q], r, ux, uy;
-  active_node(a, an, out, r, ux, uy);
+  active_node(a, fin, plane, an, out, r, ux, uy);
   const int g = an_gid[an];
 #pragma unroll
   for (int q = 0; q < Q; ++q) npdf[(size_t)q * a.N + g] = out[q];
@@ -411,8 +623,10 @@ __global__ void k_cell_moments(StepArgs<T> a, T *__restrict__ rho, T *__restrict
                                T *__restrict__ feq) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.C) return;
+  const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
+  const Buf fin(a.fsrc, Q * plane);
   T f[Q], eq[Q], r, ux, uy;
-  load_cell(a.fsrc, a.Cp, c, f);
+  load_cell(fin, plane, c, f);
   macro(f, r, ux, uy);
   equilibrium(a.k, r, ux, uy, eq);
   rho[c] = r;
@@ -436,6 +650,8 @@ template <typename T, int SCHEME>
 __global__ void k_face_flux(StepArgs<T> a, FaceArrays fa, T *__restrict__ flux) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= fa.F) return;
+  const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
+  const Buf fin(a.fsrc, Q * plane);
   const int ia = fa.c0[j], ib = fa.c1[j];
   const T d0 = static_cast<const T *>(fa.d0)[j], d1 = static_cast<const T *>(fa.d1)[j];
   const T nx = static_cast<const T *>(fa.nx)[j], ny = static_cast<const T *>(fa.ny)[j];
@@ -445,14 +661,14 @@ __global__ void k_face_flux(StepArgs<T> a, FaceArrays fa, T *__restrict__ flux) 
     const int idx = side == 0 ? ia : ib;
     T tmp[Q];
     if (idx != -1) {
-      load_cell(a.fsrc, a.Cp, idx, tmp);
+      load_cell(fin, plane, idx, tmp);
     } else {
       int known = side == 0 ? ib : ia;
       if (known < 0) known = fa.last_cell;
       const T gd = side == 0 ? d0 : d1, kd = side == 0 ? d1 : d0;
       const T ratio = gd / kd;
       T fk[Q];
-      load_cell(a.fsrc, a.Cp, known, fk);
+      load_cell(fin, plane, known, fk);
       const int p = fa.n0[j], r = fa.n1[j];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
@@ -505,6 +721,15 @@ __global__ void k_residual_finalize(const double *__restrict__ partials, int nti
   }
 }
 
+// Plain device copy (16 B per lane per iteration, grid-stride): the
+// known-size reference for converting FETCH_SIZE / WRITE_SIZE counters and
+// for the attainable copy bandwidth on this device.
+__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template <typename T>
 __global__ void k_fill_uniform(T *__restrict__ f, int Cp, int C, Lattice<T> k) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -536,11 +761,11 @@ struct Handle {
   std::vector<int32_t> an_gid_host, an_type_host;
   // device pointers (typed as void to share between precisions)
   void *f[2] = {nullptr, nullptr};
-  void *nb = nullptr, *nx = nullptr, *ny = nullptr, *sL = nullptr, *lwa = nullptr,
-       *lwden2 = nullptr;
+  void *geo = nullptr, *lwa = nullptr, *lwden2 = nullptr;
   void *bf_slot = nullptr, *bf_gnode = nullptr, *bf_ratio = nullptr;
   void *tile_node_off = nullptr, *tile_nodes = nullptr;
-  int max_tile_nodes = 0;
+  void *tile_pair_off = nullptr, *entry_pair_off = nullptr, *pair_cell = nullptr, *pair_w = nullptr;
+  int max_tile_nodes = 0, max_tile_pairs = 0;
   void *an_gid = nullptr, *an_type = nullptr, *an_cells = nullptr, *an_w = nullptr,
        *an_wsum = nullptr, *an_rho = nullptr, *an_ux = nullptr, *an_uy = nullptr;
   void *node_pdf = nullptr, *node_rho = nullptr, *node_vel = nullptr;
@@ -596,16 +821,17 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   StepArgs<T> a{};
   a.fsrc = static_cast<const T *>(h.f[src]);
   a.fdst = static_cast<T *>(h.f[src ^ 1]);
-  a.nb = static_cast<const int32_t *>(h.nb);
-  a.nx = static_cast<const T *>(h.nx);
-  a.ny = static_cast<const T *>(h.ny);
-  a.sL = static_cast<const T *>(h.sL);
+  a.geo = h.geo;
   a.lwa = static_cast<const T *>(h.lwa);
   a.lwden2 = static_cast<const T *>(h.lwden2);
   a.bf_slot = static_cast<const int32_t *>(h.bf_slot);
   a.bf_gnode = static_cast<const int32_t *>(h.bf_gnode);
   a.tile_node_off = static_cast<const int32_t *>(h.tile_node_off);
   a.tile_nodes = static_cast<const int32_t *>(h.tile_nodes);
+  a.tile_pair_off = static_cast<const int32_t *>(h.tile_pair_off);
+  a.entry_pair_off = static_cast<const int32_t *>(h.entry_pair_off);
+  a.pair_cell = static_cast<const int32_t *>(h.pair_cell);
+  a.pair_w = static_cast<const T *>(h.pair_w);
   a.bf_ratio = static_cast<const T *>(h.bf_ratio);
   a.an_type = static_cast<const int32_t *>(h.an_type);
   a.an_cells = static_cast<const int32_t *>(h.an_cells);
@@ -625,6 +851,9 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.ntiles = h.ntiles;
   a.xcd_remap = (h.variant & 1) ? 0 : 1;
   a.inline_nodes = (h.variant & 2) ? 0 : 1;
+  a.stage = (h.variant & 4) ? 0 : 1;
+  a.max_tile_nodes = h.max_tile_nodes;
+  a.max_tile_pairs = h.max_tile_pairs;
   a.k = make_lattice<T>(h.tau, h.dt);
   return a;
 }
@@ -645,7 +874,9 @@ static int launch_step(Handle &h, int src, bool resid) {
                        static_cast<T *>(h.node_rho), static_cast<T *>(h.node_vel));
   }
   const dim3 grid(grid_for(h, a.xcd_remap)), block(h.block);
-  const size_t lds = a.inline_nodes ? (size_t)Q * h.max_tile_nodes * sizeof(T) : 0;
+  const size_t lds =
+      ((size_t)Q * (h.max_tile_nodes + (a.stage ? h.block : 0)) + 12 * (size_t)h.max_tile_pairs) *
+      sizeof(T);
   if (h.scheme == FVDBM_UPWIND) {
     if (resid)
       hipLaunchKernelGGL((k_step<T, 0, true>), grid, block, lds, h.stream, a);
@@ -731,26 +962,53 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
   h.nan = L.nan;
   h.bandwidth = L.bandwidth;
   h.last_cell_new = L.last_cell_new;
+  // raw buffer descriptors address at most 2^32 - 1 bytes
+  if ((uint64_t)Q * L.Cp * sizeof(T) >= (1ull << 32) ||
+      (uint64_t)3 * L.Cp * GeoLayout<T>::bytes >= (1ull << 32))
+    return fail(FVDBM_ERR_INVALID, "mesh too large for one handle (pdf array must be < 4 GiB)");
   h.ntiles = L.ntiles;
   h.max_tile_nodes = L.max_tile_nodes;
-  // node tables beyond 64 KiB of LDS per block: use the two-kernel variant
-  if ((size_t)Q * L.max_tile_nodes * sizeof(T) > 65536) h.variant |= 2;
+  h.max_tile_pairs = L.max_tile_pairs;
+  // LDS per block is capped at 64 KiB: node tables beyond 40 KiB use the
+  // two-kernel variant, and the pdf staging is dropped when it does not fit
+  if (((size_t)Q * L.max_tile_nodes + 12 * (size_t)L.max_tile_pairs) * sizeof(T) > 40960) {
+    h.variant |= 2;
+    h.max_tile_nodes = 0;
+    h.max_tile_pairs = 0;
+  }
+  if (((size_t)Q * (h.max_tile_nodes + h.block) + 12 * (size_t)h.max_tile_pairs) * sizeof(T) >
+      65536)
+    h.variant |= 4;
   h.new_to_old = L.new_to_old;
   h.an_gid_host = L.an_gid;
   h.an_type_host = L.an_type;
   const size_t es = sizeof(T);
   int rc = 0;
   for (int i = 0; i < 2 && !rc; ++i) rc = h.alloc(&h.f[i], (size_t)Q * h.Cp * es);
-  if (!rc) rc = h.upload(&h.nb, L.nb);
-  if (!rc) rc = h.upload(&h.nx, L.nx);
-  if (!rc) rc = h.upload(&h.ny, L.ny);
-  if (!rc) rc = h.upload(&h.sL, L.sL);
+  if (!rc) {
+    // interleave (code, nx, ny, sL) per cell face into GeoLayout<T> records
+    const size_t rec = GeoLayout<T>::bytes, n = 3 * (size_t)L.Cp;
+    std::vector<unsigned char> g(n * rec, 0);
+    for (size_t i = 0; i < n; ++i) {
+      unsigned char *d = g.data() + i * rec;
+      std::memcpy(d, &L.nb[i], 4);
+      const size_t o = sizeof(T) == 4 ? 4 : 8;
+      std::memcpy(d + o, &L.nx[i], sizeof(T));
+      std::memcpy(d + o + sizeof(T), &L.ny[i], sizeof(T));
+      std::memcpy(d + o + 2 * sizeof(T), &L.sL[i], sizeof(T));
+    }
+    rc = h.upload(&h.geo, g);
+  }
   if (!rc && h.scheme == FVDBM_LAX_WENDROFF) rc = h.upload(&h.lwa, L.lwa);
   if (!rc && h.scheme == FVDBM_LAX_WENDROFF) rc = h.upload(&h.lwden2, L.lwden2);
   if (!rc) rc = h.upload(&h.bf_slot, L.bf_slot);
   if (!rc) rc = h.upload(&h.bf_gnode, L.bf_gnode);
   if (!rc) rc = h.upload(&h.tile_node_off, L.tile_node_off);
   if (!rc) rc = h.upload(&h.tile_nodes, L.tile_nodes);
+  if (!rc) rc = h.upload(&h.tile_pair_off, L.tile_pair_off);
+  if (!rc) rc = h.upload(&h.entry_pair_off, L.entry_pair_off);
+  if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
+  if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
   if (!rc) rc = h.upload(&h.bf_ratio, L.bf_ratio);
   if (!rc) rc = h.upload(&h.an_gid, L.an_gid);
   if (!rc) rc = h.upload(&h.an_type, L.an_type);
@@ -994,7 +1252,7 @@ int fvdbm_device_count(int32_t *count) {
 
 int fvdbm_variant_count(int32_t *count) {
   if (!count) return fail(FVDBM_ERR_INVALID, "null count");
-  *count = 4;  // bit 0: no XCD remap; bit 1: separate node kernel
+  *count = 8;  // bit 0: no XCD remap; bit 1: separate node kernel; bit 2: no LDS staging
   return FVDBM_OK;
 }
 
@@ -1010,7 +1268,7 @@ int fvdbm_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, void **out)
   int block = p->block ? p->block : 256;
   if (block < 64 || block > 1024 || block % 64)
     return fail(FVDBM_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
-  if (p->variant < 0 || p->variant > 3) return fail(FVDBM_ERR_INVALID, "variant out of range");
+  if (p->variant < 0 || p->variant > 7) return fail(FVDBM_ERR_INVALID, "variant out of range");
   int gs = p->graph_steps == 0 ? 32 : p->graph_steps;
   if (gs > 0 && (gs % 2)) ++gs;  // even, so a replay preserves buffer parity
   std::unique_ptr<Handle> h(new Handle());
@@ -1134,6 +1392,46 @@ int fvdbm_synchronize(void *handle) {
   Handle *h = static_cast<Handle *>(handle);
   if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return FVDBM_OK;
+}
+
+int fvdbm_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy) {
+  if (bytes < 16 || reps < 1 || !ms_per_copy) return fail(FVDBM_ERR_INVALID, "bad copy probe arguments");
+  const size_t n = (size_t)bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  hipError_t e = hipMalloc(&src, n * 16);
+  if (e == hipSuccess) e = hipMalloc(&dst, n * 16);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 0x3c, n * 16, st);
+  const int grid = 256 * 8 * 4;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fvdbm::k_copy_probe, dim3(grid), dim3(256), 0, st,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+    e = hipEventRecord(e0, st);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipLaunchKernelGGL(fvdbm::k_copy_probe, dim3(grid), dim3(256), 0, st,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  cleanup();
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+  *ms_per_copy = t / reps;
   return FVDBM_OK;
 }
 

@@ -39,7 +39,49 @@ int bfs_last_level(int s, const std::vector<int32_t> &xadj,
   return depth;
 }
 
+// Index of (x, y) along a Hilbert curve over a 2^order x 2^order grid.
+uint64_t hilbert_index(uint32_t x, uint32_t y, int order) {
+  uint64_t d = 0;
+  for (uint32_t s = 1u << (order - 1); s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {  // rotate the quadrant
+      if (rx == 1) {
+        x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
+        y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
 }  // namespace
+
+std::vector<int32_t> hilbert_order(int n, const double *xy) {
+  double x0 = xy[0], x1 = xy[0], y0 = xy[1], y1 = xy[1];
+  for (int i = 0; i < n; ++i) {
+    x0 = std::min(x0, xy[2 * i]);
+    x1 = std::max(x1, xy[2 * i]);
+    y0 = std::min(y0, xy[2 * i + 1]);
+    y1 = std::max(y1, xy[2 * i + 1]);
+  }
+  const int order = 20;
+  const double span = std::max(x1 - x0, y1 - y0);
+  const double scale = span > 0 ? ((1u << order) - 1) / span : 0.0;
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t gx = (uint32_t)((xy[2 * i] - x0) * scale);
+    const uint32_t gy = (uint32_t)((xy[2 * i + 1] - y0) * scale);
+    key[i] = {hilbert_index(gx, gy, order), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+  return perm;
+}
 
 std::vector<int32_t> reverse_cuthill_mckee(int n, const std::vector<int32_t> &xadj,
                                            const std::vector<int32_t> &adj) {
@@ -165,8 +207,21 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
 
   // --- cell order
   std::vector<int32_t> perm(C);
+  if (reorder < 0 || reorder > 4) {
+    err = "reorder must be 0, 1, 2, 3 or 4";
+    return false;
+  }
+  if (reorder >= 3 && !m.cell_xy) {
+    err = "reorder 3/4 (Hilbert order) needs cell centroids (cell_xy)";
+    return false;
+  }
   if (reorder == 0) {
     std::iota(perm.begin(), perm.end(), 0);
+  } else if (reorder >= 3) {
+    perm = hilbert_order(C, m.cell_xy);
+    if (reorder == 3)
+      std::stable_partition(perm.begin(), perm.end(),
+                            [&](int32_t c) { return has_ghost[c] != 0; });
   } else {
     std::vector<int32_t> xadj(C + 1, 0), adj;
     adj.reserve(3 * (size_t)C);
@@ -306,6 +361,31 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
   L.max_tile_nodes = 0;
   for (int t = 0; t < L.ntiles; ++t)
     L.max_tile_nodes = std::max(L.max_tile_nodes, L.tile_node_off[t + 1] - L.tile_node_off[t]);
+  // (node, cell) pairs per tile table entry
+  L.tile_pair_off.assign(L.ntiles + 1, 0);
+  L.entry_pair_off.assign(L.tile_nodes.size() + 1, 0);
+  L.pair_cell.clear();
+  L.pair_w.clear();
+  L.max_tile_pairs = 0;
+  for (int t = 0; t < L.ntiles; ++t) {
+    L.tile_pair_off[t] = (int32_t)L.pair_cell.size();
+    for (int e = L.tile_node_off[t]; e < L.tile_node_off[t + 1]; ++e) {
+      L.entry_pair_off[e] = (int32_t)L.pair_cell.size();
+      const int an = L.tile_nodes[e];
+      if (an < 0) continue;
+      for (int k = 0; k < K; ++k) {
+        const int32_t cj = L.an_cells[(size_t)an * K + k];
+        const T w = L.an_w[(size_t)an * K + k];
+        if (cj < 0 && w == T(0)) continue;
+        L.pair_cell.push_back(cj);
+        L.pair_w.push_back(w);
+      }
+    }
+    L.max_tile_pairs =
+        std::max(L.max_tile_pairs, (int)L.pair_cell.size() - L.tile_pair_off[t]);
+  }
+  L.tile_pair_off[L.ntiles] = (int32_t)L.pair_cell.size();
+  L.entry_pair_off[L.tile_nodes.size()] = (int32_t)L.pair_cell.size();
   L.nbf = (int)L.bf_ratio.size();
   L.bf_gnode.resize(2 * (size_t)L.nbf);
   L.bf_slot.resize(2 * (size_t)L.nbf);

@@ -46,6 +46,16 @@ struct HostLayout {
   int ntiles = 0, max_tile_nodes = 0;
   std::vector<int32_t> tile_node_off, tile_nodes;
 
+  // (node, cell) pairs of the active nodes in each tile table, in table
+  // order and, per node, in nodes.cells_index order; padding entries with
+  // weight 0 are left out (they add +0). pair_cell: device cell id, or
+  // -2 - (wrapped id) for a padding entry with a non-zero weight.
+  int max_tile_pairs = 0;
+  std::vector<int32_t> tile_pair_off;   // [ntiles+1]
+  std::vector<int32_t> entry_pair_off;  // [tile_nodes.size()+1], absolute
+  std::vector<int32_t> pair_cell;
+  std::vector<T> pair_w;
+
   // active nodes
   std::vector<int32_t> an_gid, an_type;
   std::vector<int32_t> an_cells;  // [nan][K]: device id, or -2 - (wrapped id) for padding
@@ -63,6 +73,9 @@ struct HostLayout {
 template <typename T>
 bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
                   HostLayout<T> &out, std::string &err);
+
+// Cells sorted by the Hilbert index of their centroids ([n][2]).
+std::vector<int32_t> hilbert_order(int n, const double *xy);
 
 // Reverse Cuthill-McKee over an undirected graph in CSR form.
 // Returns perm with perm[new] = old.

@@ -50,7 +50,7 @@ class Environment:
     dynamics = None  # class attribute used by create(), as in the reference
 
     def __init__(self, cells: Cells, faces: Faces, nodes: Nodes, precision="fp32",
-                 reorder=1, variant=0, block=0, graph_steps=0):
+                 reorder=None, variant=0, block=0, graph_steps=0):
         self.cells = cells
         self.faces = faces
         self.nodes = nodes
@@ -101,7 +101,8 @@ class Environment:
             face_nodes=f.nodes_index, face_n=face_n,
             face_L=np.asarray(face_L).reshape(-1), node_cells=n.cells_index,
             node_dists=node_dists, node_type=np.asarray(n.type).reshape(N),
-            node_rho=np.asarray(node_rho).reshape(N), node_vel=np.asarray(node_vel).reshape(N, 2))
+            node_rho=np.asarray(node_rho).reshape(N), node_vel=np.asarray(node_vel).reshape(N, 2),
+            cell_xy=getattr(c, "centers", None))
 
     def init(self):
         """Convert the ragged arrays, upload the static geometry and the
@@ -114,12 +115,15 @@ class Environment:
         self.close()
         dt = self.dtype
         arrays = self._mesh_arrays()
+        reorder = self.options["reorder"]
+        if reorder is None:  # Hilbert order when centroids are known, else RCM
+            reorder = 4 if arrays.get("cell_xy") is not None else 1
         desc = _lib.mesh_desc(arrays, dt)
         dyn = self.cells.dynamics
         p = _lib.Params(float(dyn.tau), float(dyn.delta_t),
                         _lib.FP64 if self.precision == "fp64" else _lib.FP32,
                         _lib.LAX_WENDROFF if self.faces.flux_scheme == "lax_wendroff" else _lib.UPWIND,
-                        int(self.options["reorder"]), int(self.options["variant"]),
+                        int(reorder), int(self.options["variant"]),
                         int(self.options["block"]), int(self.options["graph_steps"]))
         h = ct.c_void_p()
         _lib.check(_lib.lib().fvdbm_create(ct.byref(desc), ct.byref(p), ct.byref(h)), "fvdbm_create")

@@ -61,6 +61,21 @@ class Mesher:
     """Constructs the static geometry of a 2D triangular mesh
     (reference: src/mesher.py:20-45)."""
 
+    def _face_cells_raw(self):
+        """(c0, c1, count) per face from the (cell, local edge) table; the
+        sorted-pair key ignores face orientation, so one evaluation serves
+        every pass of calc_mesh_properties."""
+        cache = self.__dict__.get("_fc_cache")
+        if cache is not None and cache[0] is self.cells and cache[1] == len(self.faces):
+            return cache[2]
+        npts = self.points.shape[0]
+        f = self.faces
+        fkey = np.minimum(f[:, 0], f[:, 1]).astype(np.int64) * npts + np.maximum(f[:, 0], f[:, 1])
+        ekey, eown = _edge_table(self.cells, npts)
+        res = _first_two_cells(fkey, ekey, eown)
+        self.__dict__["_fc_cache"] = (self.cells, len(self.faces), res)
+        return res
+
     def __init__(self):
         self.points = None
         self.cells = None
@@ -110,11 +125,8 @@ class Mesher:
     def enforce_boundary_face_normals_outward(self):
         """Flip boundary faces whose left normal points into their cell
         (src/mesher.py:80-110)."""
-        npts = self.points.shape[0]
-        ekey, eown = _edge_table(self.cells, npts)
         f = self.faces
-        fkey = np.minimum(f[:, 0], f[:, 1]).astype(np.int64) * npts + np.maximum(f[:, 0], f[:, 1])
-        c0, _, cnt = _first_two_cells(fkey, ekey, eown)
+        c0, _, cnt = self._face_cells_raw()
         bnd = np.nonzero(cnt == 1)[0]
         p0 = self.points[f[bnd, 0]]
         p1 = self.points[f[bnd, 1]]
@@ -177,11 +189,7 @@ class Mesher:
         """Cell-to-cell (interior) or cell-to-face (boundary) unit vectors,
         flipped to align with the face normal (src/mesher.py:506-544). Only
         the cell-centred stencil variants read them."""
-        npts = self.points.shape[0]
-        f = self.faces
-        fkey = np.minimum(f[:, 0], f[:, 1]).astype(np.int64) * npts + np.maximum(f[:, 0], f[:, 1])
-        ekey, eown = _edge_table(self.cells, npts)
-        c0, c1, cnt = _first_two_cells(fkey, ekey, eown)
+        c0, c1, cnt = self._face_cells_raw()
         v = np.zeros_like(self.face_normals)
         two = cnt == 2
         v[two] = self.cell_centers[c1[two]] - self.cell_centers[c0[two]]
@@ -199,11 +207,8 @@ class Mesher:
         """Slot 0 = the cell behind the face normal (smaller n.(centroid -
         midpoint)); boundary faces are [cell, -1]; distances are |n.(x_c -
         x_f)| (src/mesher.py:197-266)."""
-        npts = self.points.shape[0]
         f = self.faces
-        fkey = np.minimum(f[:, 0], f[:, 1]).astype(np.int64) * npts + np.maximum(f[:, 0], f[:, 1])
-        ekey, eown = _edge_table(self.cells, npts)
-        c0, c1, _ = _first_two_cells(fkey, ekey, eown)
+        c0, c1, _ = self._face_cells_raw()
         mid = (self.points[f[:, 0]] + self.points[f[:, 1]]) / 2.0
         n = self.face_normals
         cen = self.cell_centers
@@ -354,6 +359,7 @@ class Mesher:
         cells = Cells(C, dynamics)
         cells.face_indices = np.asarray(self.cell_face_indices, np.int32)
         cells.face_normals = np.asarray(self.cell_face_normal_signs, np.int32)
+        cells.centers = np.asarray(self.cell_centers, np.float64)  # for the device cell order
 
         faces = Faces(F, dynamics, flux_scheme=flux_method)
         faces.n = self.face_normals.astype(np.float32)

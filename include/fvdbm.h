@@ -75,6 +75,9 @@ typedef struct fvdbm_mesh_desc {
   const int32_t *node_type;  /* [N]    nodes.type: 0 none, 1 velocity, 2 density */
   const void *node_rho;      /* [N]    nodes.rho  (density-node values) */
   const void *node_vel;      /* [N][2] nodes.vel  (velocity-node values) */
+  /* Optional (may be NULL): cell centroids [C][2] (Mesher.cell_centers),
+   * used only to order cells along a Hilbert curve (reorder = 3). */
+  const double *cell_xy;
 } fvdbm_mesh_desc;
 
 /* D2Q9(tau, delta_t) (src/dynamics.py:65-68) and layout/launch options. */
@@ -83,7 +86,10 @@ typedef struct fvdbm_params {
   double delta_t;
   int32_t precision;  /* FVDBM_FP32 | FVDBM_FP64 */
   int32_t scheme;     /* FVDBM_UPWIND | FVDBM_LAX_WENDROFF (Faces.flux_scheme) */
-  int32_t reorder;    /* 1: reverse Cuthill-McKee cell order (boundary cells first) */
+  int32_t reorder;    /* device cell order: 0 reference order, 1 reverse Cuthill-McKee
+                         with ghost-face cells first, 2 RCM, 3 Hilbert curve of the
+                         centroids with ghost-face cells first, 4 Hilbert curve
+                         (3 and 4 need cell_xy) */
   int32_t variant;    /* step kernel variant, 0 = default (see fvdbm_variant_count) */
   int32_t block;      /* threads per block (64..1024, multiple of 64), 0 = default */
   int32_t graph_steps;/* steps per captured hipGraph, 0 = default, <0 = no graphs */
@@ -132,13 +138,19 @@ int fvdbm_step_residual(void *handle, int64_t nsteps, double *residual);
 int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms);
 int fvdbm_synchronize(void *handle);
 
+/* Measurement aid (no reference counterpart): copies `bytes` between two
+ * fresh device buffers `reps` times with a plain 16-byte-per-lane kernel
+ * (k_copy_probe) and returns milliseconds per copy. Used to scale the
+ * FETCH_SIZE / WRITE_SIZE counters and as the attainable-bandwidth check. */
+int fvdbm_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* Device cell order: new_to_old[i] = reference cell id of device cell i. */
 int fvdbm_get_permutation(void *handle, int32_t *new_to_old);
 
 /* Host-only (no device needed): the cell order fvdbm_create would use for
- * this mesh with `reorder` (0 identity, 1 RCM with ghost-face cells first,
- * 2 RCM), and stats = {boundary cells, boundary faces, active nodes,
- * bandwidth}. Returns FVDBM_ERR_INVALID for an inconsistent mesh. */
+ * this mesh with `reorder` (see fvdbm_params), and stats = {boundary cells,
+ * boundary faces, active nodes, bandwidth}. Returns FVDBM_ERR_INVALID for an
+ * inconsistent mesh. */
 int fvdbm_layout_order(const fvdbm_mesh_desc *mesh, int32_t precision, int32_t reorder,
                        int32_t *new_to_old, int32_t *stats);
 

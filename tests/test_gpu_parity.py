@@ -102,7 +102,8 @@ def test_channel_matches_oracle(gpu, scheme):
 
 
 @pytest.mark.parametrize("opts", [
-    dict(reorder=0), dict(reorder=2), dict(variant=1), dict(variant=2), dict(variant=3),
+    dict(reorder=0), dict(reorder=1), dict(reorder=2), dict(reorder=3), dict(variant=1), dict(variant=2),
+    dict(variant=3), dict(variant=4), dict(variant=7), dict(reorder=3, block=1024),
     dict(block=64), dict(block=512), dict(block=1024), dict(graph_steps=-1), dict(graph_steps=6),
 ])
 def test_layout_and_variants_bitwise_equal(gpu, opts):
