@@ -134,6 +134,10 @@ def main():
         cms = _lib.copy_probe(nbytes, 20)
         copy = dict(bytes=nbytes, ms=round(cms, 5), GBps=round(2 * nbytes / (cms / 1e3) / 1e9, 1))
         log(f"copy probe: 256 MiB in {cms:.4f} ms -> {copy['GBps']} GB/s (read + write)")
+        for mib in (16, 32, 64, 128):
+            t = _lib.copy_probe(mib << 20, 50)
+            copy[f"GBps_{mib}MiB"] = round(2 * (mib << 20) / (t / 1e3) / 1e9, 1)
+        log(f"copy probe by size: {copy}")
     fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache)
     C = len(m.cells)
     env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block,

@@ -146,9 +146,12 @@ template <>
 __device__ __forceinline__ EqTerms<float> eq_terms<float>(const Lattice<float> &k, float ux,
                                                           float uy, float uu, float c5, float c6) {
   const float s1 = ux * ux, s2 = uy * uy, s5 = c5 * c5, s6 = c6 * c6;
-  const bool ok = fast_div_ok(uu) & fast_div_ok(ux) & fast_div_ok(uy) & fast_div_ok(c5) &
-                  fast_div_ok(c6) & fast_div_ok(s1) & fast_div_ok(s2) & fast_div_ok(s5) &
-                  fast_div_ok(s6);
+  // biased exponents of ux, uy, c5, c6 in [84, 188] put them and the five
+  // squares and sums derived from them (uu >= ux*ux) inside [40, 251]
+  auto ex = [](float v) { return (__float_as_uint(v) >> 23) & 0xffu; };
+  const uint32_t emin = min(min(ex(ux), ex(uy)), min(ex(c5), ex(c6)));
+  const uint32_t emax = max(max(ex(ux), ex(uy)), max(ex(c5), ex(c6)));
+  const bool ok = emin >= 84u && emax <= 188u;
   if (ok) {
     return {fast_div(uu, k.C2x2, k.rC2x2), fast_div(ux, k.C2, k.rC2),
             fast_div(uy, k.C2, k.rC2),     fast_div(c5, k.C2, k.rC2),
@@ -172,15 +175,17 @@ __device__ __forceinline__ void equilibrium(const Lattice<T> &k, T rho, T ux, T 
   const T D = t.D, A1 = t.A1, A2 = t.A2, A5 = t.A5, A6 = t.A6;
   const T B1 = t.B1, B2 = t.B2, B5 = t.B5, B6 = t.B6;
   const T one = T(1);
-  feq[0] = (k.W[0] * rho) * (((one + T(0)) + T(0)) - D);
-  feq[1] = (k.W[1] * rho) * (((one + A1) + B1) - D);
-  feq[2] = (k.W[2] * rho) * (((one + A2) + B2) - D);
-  feq[3] = (k.W[3] * rho) * (((one - A1) + B1) - D);
-  feq[4] = (k.W[4] * rho) * (((one - A2) + B2) - D);
-  feq[5] = (k.W[5] * rho) * (((one + A5) + B5) - D);
-  feq[6] = (k.W[6] * rho) * (((one + A6) + B6) - D);
-  feq[7] = (k.W[7] * rho) * (((one - A5) + B5) - D);
-  feq[8] = (k.W[8] * rho) * (((one - A6) + B6) - D);
+  // W has three distinct values (W[1..4] and W[5..8] are equal floats)
+  const T wr0 = k.W[0] * rho, wr1 = k.W[1] * rho, wr5 = k.W[5] * rho;
+  feq[0] = wr0 * (((one + T(0)) + T(0)) - D);
+  feq[1] = wr1 * (((one + A1) + B1) - D);
+  feq[2] = wr1 * (((one + A2) + B2) - D);
+  feq[3] = wr1 * (((one - A1) + B1) - D);
+  feq[4] = wr1 * (((one - A2) + B2) - D);
+  feq[5] = wr5 * (((one + A5) + B5) - D);
+  feq[6] = wr5 * (((one + A6) + B6) - D);
+  feq[7] = wr5 * (((one - A5) + B5) - D);
+  feq[8] = wr5 * (((one - A6) + B6) - D);
 }
 
 // ----------------------------------------------------------------------------
@@ -280,6 +285,7 @@ struct StepArgs {
   const T *__restrict__ an_uy;
   const T *__restrict__ node_pdf;  // [9][N]
   double *__restrict__ partials;   // [ntiles][2] residual partial sums
+  const int32_t *__restrict__ block_tile;  // [grid] tile of each block (XCD-aware), -1 = none
   int C, Cp, N, K, nbf, nan;
   int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs;
   Lattice<T> k;
@@ -357,12 +363,14 @@ __device__ __forceinline__ void active_node(const StepArgs<T> &a, const Buf &fb,
 }
 
 // XCD-aware tile order: consecutive workgroup ids go round-robin to the 8
-// XCDs, so give each XCD one contiguous range of tiles; neighbour gathers
-// then mostly hit that XCD's own L2.
-__device__ __forceinline__ int tile_of_block(int b, int nblocks, int remap) {
-  if (!remap) return b;
-  const int per = nblocks >> 3;  // grid is a multiple of 8 when remapping
-  return (b & 7) * per + (b >> 3);
+// XCDs, so each XCD gets one contiguous range of tiles (neighbour gathers then
+// mostly hit that XCD's own L2); inside a range the tiles with boundary-node
+// work come first, so their longer latency overlaps the bulk of the grid.
+// block_tile is built on the host; without it the identity order is used.
+template <typename T>
+__device__ __forceinline__ int tile_of_block(const StepArgs<T> &a) {
+  if (!a.xcd_remap) return blockIdx.x;
+  return a.block_tile[blockIdx.x];
 }
 
 template <typename T>
@@ -404,8 +412,8 @@ template <typename T, int SCHEME, bool RESID>
 __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   T *sh = reinterpret_cast<T *>(smem_raw);
-  const int tile = tile_of_block(blockIdx.x, gridDim.x, a.xcd_remap);
-  if (tile >= a.ntiles) return;  // uniform per block
+  const int tile = tile_of_block(a);
+  if (tile < 0 || tile >= a.ntiles) return;  // uniform per block
   const uint32_t plane = (uint32_t)a.Cp * sizeof(T);  // bytes between q planes
   const Buf fin(a.fsrc, Q * plane);
   const int t0 = a.tile_node_off[tile];
@@ -547,15 +555,20 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
         }
       }
       if constexpr (SCHEME == 0) {
+        // ksi.n for q = 1..8. Opposite directions give exact negatives, so
+        // two sums cover all eight; the value can differ from the literal
+        // 0-initialised dot product only in the sign of a zero, which makes
+        // only the sign of a zero flux differ, and an accumulator started at
+        // +0 absorbs a zero of either sign identically. q = 0 has ksi = 0:
+        // its flux is a zero and is never added.
+        const T p5 = nx + ny, p6 = ny - nx;
+        const T phi[Q] = {T(0), nx, ny, -nx, -ny, p5, p6, -p5, -p6};
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          T phi = T(0);
-          phi += T(kx(q)) * nx;
-          phi += T(ky(q)) * ny;
+        for (int q = 1; q < Q; ++q) {
           // upwind: slot 0 when ksi.n >= 0, else slot 1
-          const bool take_slot0 = phi >= T(0);
+          const bool take_slot0 = phi[q] >= T(0);
           const T up = (take_slot0 != slot1) ? f[q] : o[q];
-          acc[q] += (up * phi) * sL;
+          acc[q] += (up * phi[q]) * sL;
         }
       } else {
         const size_t at = (size_t)kf * a.Cp + c;
@@ -766,6 +779,7 @@ struct Handle {
   void *tile_node_off = nullptr, *tile_nodes = nullptr;
   void *tile_pair_off = nullptr, *entry_pair_off = nullptr, *pair_cell = nullptr, *pair_w = nullptr;
   int max_tile_nodes = 0, max_tile_pairs = 0;
+  void *block_tile = nullptr;
   void *an_gid = nullptr, *an_type = nullptr, *an_cells = nullptr, *an_w = nullptr,
        *an_wsum = nullptr, *an_rho = nullptr, *an_ux = nullptr, *an_uy = nullptr;
   void *node_pdf = nullptr, *node_rho = nullptr, *node_vel = nullptr;
@@ -842,6 +856,7 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.an_uy = static_cast<const T *>(h.an_uy);
   a.node_pdf = static_cast<const T *>(h.node_pdf);
   a.partials = static_cast<double *>(h.partials);
+  a.block_tile = static_cast<const int32_t *>(h.block_tile);
   a.C = h.C;
   a.Cp = h.Cp;
   a.N = h.N;
@@ -1009,6 +1024,23 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
   if (!rc) rc = h.upload(&h.entry_pair_off, L.entry_pair_off);
   if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
   if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
+  if (!rc) {
+    // block -> tile for the XCD-aware launch: block b runs on XCD b % 8 and
+    // takes the (b / 8)-th tile of that XCD's contiguous range, boundary
+    // tiles of the range first
+    const int per = (L.ntiles + 7) / 8;
+    std::vector<int32_t> bt(8 * (size_t)per, -1);
+    for (int x = 0; x < 8; ++x) {
+      const int lo = std::min(x * per, L.ntiles), hi = std::min((x + 1) * per, L.ntiles);
+      std::vector<int32_t> order;
+      for (int t = lo; t < hi; ++t)
+        if (L.tile_node_off[t + 1] > L.tile_node_off[t]) order.push_back(t);
+      for (int t = lo; t < hi; ++t)
+        if (L.tile_node_off[t + 1] == L.tile_node_off[t]) order.push_back(t);
+      for (size_t i = 0; i < order.size(); ++i) bt[i * 8 + x] = order[i];
+    }
+    rc = h.upload(&h.block_tile, bt);
+  }
   if (!rc) rc = h.upload(&h.bf_ratio, L.bf_ratio);
   if (!rc) rc = h.upload(&h.an_gid, L.an_gid);
   if (!rc) rc = h.upload(&h.an_type, L.an_type);
