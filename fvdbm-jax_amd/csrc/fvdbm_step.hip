@@ -231,6 +231,46 @@ __device__ __forceinline__ void Buf::st<double>(double v, uint32_t voff, uint32_
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, voff, soff, 0);
 }
 
+// LDS cell records written in 16-byte pieces: a cell's 9 pdfs padded to 48
+// bytes (FP32) or 80 bytes (FP64), so a neighbour is 3 (5) aligned reads.
+template <typename T>
+struct LdsCell;
+template <>
+struct LdsCell<float> {
+  static constexpr int W = 12;  // floats per cell record
+  __device__ static __forceinline__ void put(float *base, int slot, const float (&f)[Q]) {
+    float4 *d = reinterpret_cast<float4 *>(base + slot * W);
+    d[0] = make_float4(f[0], f[1], f[2], f[3]);
+    d[1] = make_float4(f[4], f[5], f[6], f[7]);
+    d[2] = make_float4(f[8], 0.f, 0.f, 0.f);
+  }
+  __device__ static __forceinline__ void get(const float *base, int slot, float (&f)[Q]) {
+    const float4 *d = reinterpret_cast<const float4 *>(base + slot * W);
+    const float4 a = d[0], b = d[1], c = d[2];
+    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+    f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+    f[8] = c.x;
+  }
+};
+template <>
+struct LdsCell<double> {
+  static constexpr int W = 10;  // doubles per cell record
+  __device__ static __forceinline__ void put(double *base, int slot, const double (&f)[Q]) {
+    double2 *d = reinterpret_cast<double2 *>(base + slot * W);
+    d[0] = make_double2(f[0], f[1]);
+    d[1] = make_double2(f[2], f[3]);
+    d[2] = make_double2(f[4], f[5]);
+    d[3] = make_double2(f[6], f[7]);
+    d[4] = make_double2(f[8], 0.0);
+  }
+  __device__ static __forceinline__ void get(const double *base, int slot, double (&f)[Q]) {
+    const double2 *d = reinterpret_cast<const double2 *>(base + slot * W);
+    const double2 a = d[0], b = d[1], c = d[2], e = d[3], g = d[4];
+    f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = c.x;
+    f[5] = c.y; f[6] = e.x; f[7] = e.y; f[8] = g.x;
+  }
+};
+
 // One face of one cell: neighbour code, face normal, sign * length.
 // FP32: one 16-byte record; FP64: two 16-byte records (code, pad, nx | ny, sL).
 template <typename T>
@@ -286,8 +326,10 @@ struct StepArgs {
   const T *__restrict__ node_pdf;  // [9][N]
   double *__restrict__ partials;   // [ntiles][2] residual partial sums
   const int32_t *__restrict__ block_tile;  // [grid] tile of each block (XCD-aware), -1 = none
+  const int32_t *__restrict__ halo_off;    // [ntiles+1]
+  const int32_t *__restrict__ halo_cells;  // out-of-tile neighbours of each tile
   int C, Cp, N, K, nbf, nan;
-  int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs;
+  int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs, max_halo;
   Lattice<T> k;
 };
 
@@ -410,7 +452,7 @@ __device__ __forceinline__ void block_reduce_store(T v0, T v1, double *out) {
 // residual partial sums of |u^{n+1}-u^n|^2 and |u^{n+1}|^2 for this tile.
 template <typename T, int SCHEME, bool RESID>
 __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sh = reinterpret_cast<T *>(smem_raw);
   const int tile = tile_of_block(a);
   if (tile < 0 || tile >= a.ntiles) return;  // uniform per block
@@ -490,45 +532,63 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
     }
   }
 
-  // the tile's own pdfs: registers, and LDS ([9][blockDim]) so that
-  // neighbours inside the tile are read from LDS instead of L2/HBM
+  // Stage: the tile's own pdfs (registers and LDS) and its halo (the
+  // out-of-tile neighbours, LDS only) as [9][bs + max_halo]; the face
+  // records then name each neighbour by its LDS slot, so no gather of
+  // phase 2 touches L2/HBM.
   const int base = tile * bs;
   const int c = base + threadIdx.x;
-  T *shf = sh + Q * a.max_tile_nodes + 12 * a.max_tile_pairs;
+  // 16-byte aligned start of the stage (after node table and contributions)
+  const int pre = (Q * a.max_tile_nodes + 12 * a.max_tile_pairs + 7) & ~7;
+  T *shf = static_cast<T *>(__builtin_assume_aligned(sh + pre, 16));
+  constexpr int CW = LdsCell<T>::W;
   T f[Q];
+  FaceRec<T> fr[3] = {};
   if (c < a.C) {
     load_cell(fin, plane, c, f);
+    // face records are loaded with the pdfs, before the barrier, so both
+    // share one memory round trip
+    const Buf geo(a.geo, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
+    const uint32_t gplane = (uint32_t)a.Cp * GeoLayout<T>::bytes;
+    const uint32_t gvo = (uint32_t)c * GeoLayout<T>::bytes;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) fr[kf] = GeoLayout<T>::load(geo, gvo, kf * gplane);
   } else {
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = T(0);
   }
   if (a.stage) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) shf[q * bs + threadIdx.x] = f[q];
+    LdsCell<T>::put(shf, threadIdx.x, f);
+    const int h0 = a.halo_off[tile], nh = a.halo_off[tile + 1] - h0;
+    for (int i = threadIdx.x; i < nh; i += bs) {
+      T g[Q];
+      load_cell(fin, plane, a.halo_cells[h0 + i], g);
+      LdsCell<T>::put(shf, bs + i, g);
+    }
   }
   if (nodes_here || a.stage) __syncthreads();
 
   T res_d = T(0), res_n = T(0);
   if (c < a.C) {
-    const Buf geo(a.geo, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
-    const uint32_t gplane = (uint32_t)a.Cp * GeoLayout<T>::bytes;
-    const uint32_t gvo = (uint32_t)c * GeoLayout<T>::bytes;
     T acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = T(0);
 #pragma unroll
     for (int kf = 0; kf < 3; ++kf) {
-      const FaceRec<T> fr = GeoLayout<T>::load(geo, gvo, kf * gplane);
-      const int32_t code = fr.code;
-      const T nx = fr.nx, ny = fr.ny, sL = fr.sL;
+      const int32_t code = fr[kf].code;
+      const T nx = fr[kf].nx, ny = fr[kf].ny, sL = fr[kf].sL;
+      T la = T(0), den2 = T(1);
+      if constexpr (SCHEME == 1) {
+        const size_t at = (size_t)kf * a.Cp + c;
+        la = a.lwa[at];
+        den2 = a.lwden2[at];
+      }
       const bool slot1 = (code & kSlot1Bit) != 0;
       T o[Q];
       if (code >= 0) {
-        const int nbr = code & kIndexMask;
-        const unsigned loc = (unsigned)(nbr - base);
-        if (a.stage && loc < (unsigned)bs) {
-#pragma unroll
-          for (int q = 0; q < Q; ++q) o[q] = shf[q * bs + loc];
+        const int nbr = code & kIndexMask;  // LDS slot when staged, else cell id
+        if (a.stage) {
+          LdsCell<T>::get(shf, nbr, o);
         } else {
           load_cell(fin, plane, nbr, o);
         }
@@ -571,8 +631,6 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
           acc[q] += (up * phi[q]) * sL;
         }
       } else {
-        const size_t at = (size_t)kf * a.Cp + c;
-        const T la = a.lwa[at], den2 = a.lwden2[at];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           T phi = T(0);
@@ -779,7 +837,8 @@ struct Handle {
   void *tile_node_off = nullptr, *tile_nodes = nullptr;
   void *tile_pair_off = nullptr, *entry_pair_off = nullptr, *pair_cell = nullptr, *pair_w = nullptr;
   int max_tile_nodes = 0, max_tile_pairs = 0;
-  void *block_tile = nullptr;
+  void *block_tile = nullptr, *halo_off = nullptr, *halo_cells = nullptr;
+  int max_halo = 0;
   void *an_gid = nullptr, *an_type = nullptr, *an_cells = nullptr, *an_w = nullptr,
        *an_wsum = nullptr, *an_rho = nullptr, *an_ux = nullptr, *an_uy = nullptr;
   void *node_pdf = nullptr, *node_rho = nullptr, *node_vel = nullptr;
@@ -857,6 +916,9 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.node_pdf = static_cast<const T *>(h.node_pdf);
   a.partials = static_cast<double *>(h.partials);
   a.block_tile = static_cast<const int32_t *>(h.block_tile);
+  a.halo_off = static_cast<const int32_t *>(h.halo_off);
+  a.halo_cells = static_cast<const int32_t *>(h.halo_cells);
+  a.max_halo = h.max_halo;
   a.C = h.C;
   a.Cp = h.Cp;
   a.N = h.N;
@@ -871,6 +933,14 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.max_tile_pairs = h.max_tile_pairs;
   a.k = make_lattice<T>(h.tau, h.dt);
   return a;
+}
+
+// dynamic LDS of k_step: node table, pair contributions, pdf stage, face table
+static size_t lds_bytes(const Handle &h, size_t es) {
+  const bool stage = (h.variant & 4) == 0;
+  size_t n = ((size_t)Q * h.max_tile_nodes + 12 * (size_t)h.max_tile_pairs + 7) & ~(size_t)7;
+  if (stage) n += (size_t)(es == 4 ? 12 : 10) * (h.block + h.max_halo);
+  return n * es;
 }
 
 static int grid_for(const Handle &h, int remap) {
@@ -889,19 +959,13 @@ static int launch_step(Handle &h, int src, bool resid) {
                        static_cast<T *>(h.node_rho), static_cast<T *>(h.node_vel));
   }
   const dim3 grid(grid_for(h, a.xcd_remap)), block(h.block);
-  const size_t lds =
-      ((size_t)Q * (h.max_tile_nodes + (a.stage ? h.block : 0)) + 12 * (size_t)h.max_tile_pairs) *
-      sizeof(T);
-  if (h.scheme == FVDBM_UPWIND) {
-    if (resid)
-      hipLaunchKernelGGL((k_step<T, 0, true>), grid, block, lds, h.stream, a);
-    else
-      hipLaunchKernelGGL((k_step<T, 0, false>), grid, block, lds, h.stream, a);
-  } else {
-    if (resid)
-      hipLaunchKernelGGL((k_step<T, 1, true>), grid, block, lds, h.stream, a);
-    else
-      hipLaunchKernelGGL((k_step<T, 1, false>), grid, block, lds, h.stream, a);
+  const size_t lds = lds_bytes(h, sizeof(T));
+  const int key = (h.scheme == FVDBM_UPWIND ? 0 : 2) + (resid ? 1 : 0);
+  switch (key) {
+    case 0: hipLaunchKernelGGL((k_step<T, 0, false>), grid, block, lds, h.stream, a); break;
+    case 1: hipLaunchKernelGGL((k_step<T, 0, true>), grid, block, lds, h.stream, a); break;
+    case 2: hipLaunchKernelGGL((k_step<T, 1, false>), grid, block, lds, h.stream, a); break;
+    default: hipLaunchKernelGGL((k_step<T, 1, true>), grid, block, lds, h.stream, a); break;
   }
   HIP_TRY(hipGetLastError());
   return FVDBM_OK;
@@ -991,9 +1055,9 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
     h.max_tile_nodes = 0;
     h.max_tile_pairs = 0;
   }
-  if (((size_t)Q * (h.max_tile_nodes + h.block) + 12 * (size_t)h.max_tile_pairs) * sizeof(T) >
-      65536)
-    h.variant |= 4;
+  h.max_halo = L.max_halo;
+  if (lds_bytes(h, sizeof(T)) > 65536) h.variant |= 4;  // the stage does not fit
+  const bool staged = (h.variant & 4) == 0;
   h.new_to_old = L.new_to_old;
   h.an_gid_host = L.an_gid;
   h.an_type_host = L.an_type;
@@ -1004,9 +1068,10 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
     // interleave (code, nx, ny, sL) per cell face into GeoLayout<T> records
     const size_t rec = GeoLayout<T>::bytes, n = 3 * (size_t)L.Cp;
     std::vector<unsigned char> g(n * rec, 0);
+    const std::vector<int32_t> &codes = staged ? L.nb_slot : L.nb;
     for (size_t i = 0; i < n; ++i) {
       unsigned char *d = g.data() + i * rec;
-      std::memcpy(d, &L.nb[i], 4);
+      std::memcpy(d, &codes[i], 4);
       const size_t o = sizeof(T) == 4 ? 4 : 8;
       std::memcpy(d + o, &L.nx[i], sizeof(T));
       std::memcpy(d + o + sizeof(T), &L.ny[i], sizeof(T));
@@ -1024,6 +1089,8 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
   if (!rc) rc = h.upload(&h.entry_pair_off, L.entry_pair_off);
   if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
   if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
+  if (!rc) rc = h.upload(&h.halo_off, L.halo_off);
+  if (!rc) rc = h.upload(&h.halo_cells, L.halo_cells);
   if (!rc) {
     // block -> tile for the XCD-aware launch: block b runs on XCD b % 8 and
     // takes the (b / 8)-th tile of that XCD's contiguous range, boundary

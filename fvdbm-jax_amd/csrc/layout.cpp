@@ -386,6 +386,44 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
   }
   L.tile_pair_off[L.ntiles] = (int32_t)L.pair_cell.size();
   L.entry_pair_off[L.tile_nodes.size()] = (int32_t)L.pair_cell.size();
+  // halos and LDS-slot codes
+  L.halo_off.assign(L.ntiles + 1, 0);
+  L.halo_cells.clear();
+  L.nb_slot = L.nb;
+  L.max_halo = 0;
+  {
+    std::vector<int32_t> h;
+    for (int t = 0; t < L.ntiles; ++t) {
+      L.halo_off[t] = (int32_t)L.halo_cells.size();
+      const int lo = t * pad, hi = std::min((t + 1) * pad, C);
+      h.clear();
+      for (int k = 0; k < 3; ++k)
+        for (int i = lo; i < hi; ++i) {
+          const int32_t code = L.nb[(size_t)k * Cp + i];
+          if (code < 0) continue;
+          const int j = code & kIndexMask;
+          if (j < lo || j >= t * pad + pad) h.push_back(j);
+        }
+      std::sort(h.begin(), h.end());
+      h.erase(std::unique(h.begin(), h.end()), h.end());
+      L.halo_cells.insert(L.halo_cells.end(), h.begin(), h.end());
+      L.max_halo = std::max(L.max_halo, (int)h.size());
+      for (int k = 0; k < 3; ++k)
+        for (int i = lo; i < hi; ++i) {
+          int32_t &code = L.nb_slot[(size_t)k * Cp + i];
+          if (code < 0) continue;
+          const int j = code & kIndexMask;
+          int slot;
+          if (j >= t * pad && j < t * pad + pad) {
+            slot = j - t * pad;
+          } else {
+            slot = pad + (int)(std::lower_bound(h.begin(), h.end(), j) - h.begin());
+          }
+          code = (code & ~kIndexMask) | slot;
+        }
+    }
+    L.halo_off[L.ntiles] = (int32_t)L.halo_cells.size();
+  }
   L.nbf = (int)L.bf_ratio.size();
   L.bf_gnode.resize(2 * (size_t)L.nbf);
   L.bf_slot.resize(2 * (size_t)L.nbf);

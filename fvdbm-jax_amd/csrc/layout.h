@@ -32,6 +32,13 @@ struct HostLayout {
 
   // per cell-face, [3][Cp]
   std::vector<int32_t> nb;
+  // same codes with the neighbour given as its slot in the tile's LDS stage
+  // (own cells 0..tile-1, then the tile's halo cells)
+  std::vector<int32_t> nb_slot;
+  // halo of each tile: distinct out-of-tile neighbour cells, ascending
+  int max_halo = 0;
+  std::vector<int32_t> halo_off, halo_cells;
+
   std::vector<T> nx, ny, sL;      // face normal, sign * L
   std::vector<T> lwa, lwden2;     // Lax-Wendroff: d0/(d0+d1), 2*(d0+d1)
 

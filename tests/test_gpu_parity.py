@@ -62,6 +62,22 @@ def test_cavity_matches_oracle(gpu, precision):
         _assert_fields_equal(env, st, FIELDS if target in (1, 1000) else FIELDS[:1], f"step {target}")
 
 
+def test_cavity_matches_committed_fixtures(gpu):
+    """The GPU path against the committed oracle snapshots (tests/golden)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ldc32_oracle.npz"))
+    m, dyn, cells, faces, nodes = ldc_case()
+    env = _gpu_env(cells, faces, nodes)
+    done = 0
+    for n in (1, 10, 100, 1000):
+        env.step(n - done)
+        done = n
+        assert np.array_equal(env.cells.pdf, z[f"pdf_f32_{n}"]), n
+    env64 = _gpu_env(cells, faces, nodes, "fp64")
+    env64.step(1000)
+    assert np.array_equal(env64.cells.pdf, z["pdf_f64_1000"])
+
+
 def test_cavity_lax_wendroff(gpu):
     m, dyn, cells, faces, nodes = ldc_case(scheme="lax_wendroff")
     oenv = oracle_env(cells, faces, nodes)

@@ -247,3 +247,21 @@ def test_lax_wendroff_uniform_flow_is_preserved():
     f0 = st["cell_pdf"].copy()
     oracle.run(env, st, dyn.tau, dyn.delta_t, 100, "f64", scheme="lax_wendroff")
     assert np.max(np.abs(st["cell_pdf"] - f0)) < 1e-13
+
+
+def test_oracle_reproduces_committed_fixtures():
+    """The oracle still produces the committed cavity snapshots bit for bit."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ldc32_oracle.npz"))
+    m, dyn, cells, faces, nodes = ldc_case()
+    env = oracle_env(cells, faces, nodes, "f32")
+    st = oracle_state(env, nodes, "f32")
+    done = 0
+    for n in (1, 10, 100, 1000):
+        oracle.run(env, st, dyn.tau, dyn.delta_t, n - done, "f32")
+        done = n
+        assert np.array_equal(st["cell_pdf"], z[f"pdf_f32_{n}"]), n
+    env64 = oracle_env(cells, faces, nodes, "f64")
+    st64 = oracle_state(env64, nodes, "f64")
+    oracle.run(env64, st64, dyn.tau, dyn.delta_t, 1000, "f64")
+    assert np.array_equal(st64["cell_pdf"], z["pdf_f64_1000"])
