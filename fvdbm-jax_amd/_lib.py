@@ -47,7 +47,15 @@ class Info(ct.Structure):
         "n_cells", "n_faces", "n_nodes", "k_max", "n_cells_padded",
         "n_boundary_cells", "n_boundary_faces", "n_active_nodes", "bandwidth",
         "precision", "scheme", "variant", "block", "graph_steps")] + [
-        ("steps_done", ct.c_int64), ("device_bytes", ct.c_int64)]
+        ("steps_done", ct.c_int64), ("device_bytes", ct.c_int64), ("n_cases", ct.c_int32)]
+
+
+class BatchDesc(ct.Structure):
+    _fields_ = [("n_cases", ct.c_int32), ("cell_case", ct.c_void_p),
+                ("case_tau", ct.c_void_p), ("case_dt", ct.c_void_p)]
+
+
+RUNNING, CONVERGED, DIVERGED, GROWING, STOPPED = range(5)
 
 
 # every function the header declares, with its ctypes signature
@@ -68,6 +76,16 @@ _SIGNATURES = {
     "fvdbm_synchronize": (ct.c_int, [ct.c_void_p]),
     "fvdbm_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "fvdbm_copy_probe": (ct.c_int, [ct.c_int64, ct.c_int32, ct.POINTER(ct.c_double)]),
+    "fvdbm_create_batch": (ct.c_int, [ct.POINTER(MeshDesc), ct.POINTER(Params), ct.POINTER(BatchDesc),
+                                      ct.POINTER(ct.c_void_p)]),
+    "fvdbm_batch_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_double, ct.c_double,
+                                   ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int64)]),
+    "fvdbm_batch_status": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_batch_set_active": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32]),
+    "fvdbm_node_fields": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_cell_fields": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_set_equilibrium": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_set_equilibrium_nodes": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "fvdbm_layout_order": (ct.c_int, [ct.POINTER(MeshDesc), ct.c_int32, ct.c_int32,
                                       ct.c_void_p, ct.c_void_p]),
 }
@@ -162,3 +180,42 @@ def copy_probe(nbytes, reps=20):
     ms = ct.c_double()
     check(lib().fvdbm_copy_probe(int(nbytes), int(reps), ct.byref(ms)), "fvdbm_copy_probe")
     return float(ms.value)
+
+
+class FieldsMixin:
+    """All-node interpolation and equilibrium initialisation, shared by the
+    single-mesh Environment and BatchSolver. Subclasses provide
+    ``_handle()``, ``dtype`` and ``_counts()`` -> (cells, nodes)."""
+
+    def node_fields(self):
+        """(rho [N], u [N,2]) at every node: inverse-distance weighting of the
+        moments of the current cell pdf."""
+        C, N = self._counts()
+        rho = np.empty(N, self.dtype)
+        vel = np.empty((N, 2), self.dtype)
+        check(lib().fvdbm_node_fields(self._handle(), ptr(rho), ptr(vel)), "fvdbm_node_fields")
+        return rho, vel
+
+    def cell_fields(self):
+        """(rho [C], u [C,2]) of the current cell pdf."""
+        C, N = self._counts()
+        rho = np.empty(C, self.dtype)
+        vel = np.empty((C, 2), self.dtype)
+        check(lib().fvdbm_cell_fields(self._handle(), ptr(rho), ptr(vel)), "fvdbm_cell_fields")
+        return rho, vel
+
+    def init_equilibrium(self, rho, vel):
+        """Cell pdf <- feq(rho, u) from cell fields rho [C], u [C,2]."""
+        C, N = self._counts()
+        r = np.ascontiguousarray(np.asarray(rho, self.dtype).reshape(C))
+        v = np.ascontiguousarray(np.asarray(vel, self.dtype).reshape(C, 2))
+        check(lib().fvdbm_set_equilibrium(self._handle(), ptr(r), ptr(v)), "fvdbm_set_equilibrium")
+
+    def init_equilibrium_nodes(self, rho, vel):
+        """Cell pdf <- feq of the node fields rho [N], u [N,2] interpolated to
+        the cells (inverse-distance weighting of each cell's vertices)."""
+        C, N = self._counts()
+        r = np.ascontiguousarray(np.asarray(rho, self.dtype).reshape(N))
+        v = np.ascontiguousarray(np.asarray(vel, self.dtype).reshape(N, 2))
+        check(lib().fvdbm_set_equilibrium_nodes(self._handle(), ptr(r), ptr(v)),
+              "fvdbm_set_equilibrium_nodes")

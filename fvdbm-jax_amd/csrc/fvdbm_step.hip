@@ -328,6 +328,16 @@ struct StepArgs {
   const int32_t *__restrict__ block_tile;  // [grid] tile of each block (XCD-aware), -1 = none
   const int32_t *__restrict__ halo_off;    // [ntiles+1]
   const int32_t *__restrict__ halo_cells;  // out-of-tile neighbours of each tile
+  const int32_t *__restrict__ tile_count;  // [ntiles] live cells (first slots of the tile)
+  // batched solves: per-case buffer parity, mask and parameters
+  const int32_t *__restrict__ tile_case;   // [ntiles]
+  const int32_t *__restrict__ case_active; // [ncases]
+  const int32_t *__restrict__ case_cur;    // [ncases] buffer holding the case's state
+  const T *__restrict__ case_invtau;
+  const T *__restrict__ case_dt;
+  const T *f0;  // the two pdf buffers (batched solves pick one per case)
+  T *f1;
+  int batch;
   int C, Cp, N, K, nbf, nan;
   int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs, max_halo;
   Lattice<T> k;
@@ -456,8 +466,21 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   T *sh = reinterpret_cast<T *>(smem_raw);
   const int tile = tile_of_block(a);
   if (tile < 0 || tile >= a.ntiles) return;  // uniform per block
+  const T *fsrc = a.fsrc;
+  T *fdst = a.fdst;
+  T invtau = a.k.invtau, dt = a.k.dt;
+  if (a.batch) {  // the tile's case: skip it when frozen, else its parity and parameters
+    const int cs = a.tile_case[tile];
+    if (!a.case_active[cs]) return;  // uniform per block
+    const int cur = a.case_cur[cs];
+    fsrc = cur ? a.f1 : a.f0;
+    fdst = cur ? const_cast<T *>(a.f0) : a.f1;
+    invtau = a.case_invtau[cs];
+    dt = a.case_dt[cs];
+  }
   const uint32_t plane = (uint32_t)a.Cp * sizeof(T);  // bytes between q planes
-  const Buf fin(a.fsrc, Q * plane);
+  const Buf fin(fsrc, Q * plane);
+  const bool live = (int)threadIdx.x < a.tile_count[tile];
   const int t0 = a.tile_node_off[tile];
   const int ntn = a.tile_node_off[tile + 1] - t0;
   const bool nodes_here = a.inline_nodes && ntn > 0;  // uniform per block
@@ -541,10 +564,9 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   // 16-byte aligned start of the stage (after node table and contributions)
   const int pre = (Q * a.max_tile_nodes + 12 * a.max_tile_pairs + 7) & ~7;
   T *shf = static_cast<T *>(__builtin_assume_aligned(sh + pre, 16));
-  constexpr int CW = LdsCell<T>::W;
   T f[Q];
   FaceRec<T> fr[3] = {};
-  if (c < a.C) {
+  if (live) {
     load_cell(fin, plane, c, f);
     // face records are loaded with the pdfs, before the barrier, so both
     // share one memory round trip
@@ -569,7 +591,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   if (nodes_here || a.stage) __syncthreads();
 
   T res_d = T(0), res_n = T(0);
-  if (c < a.C) {
+  if (live) {
     T acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = T(0);
@@ -638,7 +660,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
           phi += T(ky(q)) * ny;
           const T fu = slot1 ? o[q] : f[q];
           const T fd = slot1 ? f[q] : o[q];
-          const T interp = la - (phi * a.k.dt) / den2;
+          const T interp = la - (phi * dt) / den2;
           const T fc = fu + (fd - fu) * interp;
           acc[q] += (fc * phi) * sL;
         }
@@ -647,12 +669,12 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
     T rho, ux, uy, feq[Q];
     macro(f, rho, ux, uy);
     equilibrium(a.k, rho, ux, uy, feq);
-    const Buf fout(a.fdst, Q * plane);
+    const Buf fout(fdst, Q * plane);
     const uint32_t vo = (uint32_t)c * sizeof(T);
     T fn[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      fn[q] = f[q] + a.k.dt * (a.k.invtau * (feq[q] - f[q]) - acc[q]);
+      fn[q] = f[q] + dt * (invtau * (feq[q] - f[q]) - acc[q]);
       fout.st<T>(fn[q], vo, q * plane);
     }
     if constexpr (RESID) {
@@ -801,12 +823,163 @@ __global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ 
     dst[i] = src[i];
 }
 
+// Batched solves, after each step: active cases flip their buffer parity
+// and count the step.
+__global__ void k_batch_advance(int ncases, const int32_t *__restrict__ active,
+                                int32_t *__restrict__ cur, int64_t *__restrict__ steps) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncases || !active[k]) return;
+  cur[k] ^= 1;
+  steps[k] += 1;
+}
+
+// Batched solves, after a checked step: per case the residual of that step
+// from its tiles' partial sums (fixed order), then the case stops when it is
+// converged (r < tol), non-finite, or (growth > 0) r > growth * best r.
+// One block; n_active = number of cases still running.
+enum { kRunning = 0, kConverged = 1, kDiverged = 2, kGrowing = 3, kStopped = 4 };
+__global__ void k_batch_check(int ncases, const int32_t *__restrict__ case_tile_off,
+                              const double *__restrict__ partials, double tol, double growth,
+                              int32_t *__restrict__ active, int32_t *__restrict__ status,
+                              double *__restrict__ resid, double *__restrict__ best,
+                              int32_t *__restrict__ n_active) {
+  __shared__ int cnt[256];
+  int mine = 0;
+  for (int k = threadIdx.x; k < ncases; k += blockDim.x) {
+    if (active[k]) {
+      double d = 0.0, n = 0.0;
+      for (int t = case_tile_off[k]; t < case_tile_off[k + 1]; ++t) {
+        d += partials[2 * (size_t)t];
+        n += partials[2 * (size_t)t + 1];
+      }
+      // non-finite sums (a NaN or Inf pdf anywhere in the case) give a NaN
+      // residual, so the case stops as diverged rather than passing r < tol
+      const double r = !(isfinite(d) && isfinite(n)) ? NAN
+                       : n > 0.0                     ? sqrt(d) / sqrt(n)
+                       : d > 0.0                     ? INFINITY
+                                                     : 0.0;
+      resid[k] = r;
+      if (!isfinite(r)) {
+        status[k] = kDiverged;
+        active[k] = 0;
+      } else if (r < tol) {
+        status[k] = kConverged;
+        active[k] = 0;
+      } else if (growth > 0.0 && r > growth * best[k]) {
+        status[k] = kGrowing;
+        active[k] = 0;
+      }
+      if (r < best[k]) best[k] = r;
+    }
+    mine += active[k] ? 1 : 0;
+  }
+  cnt[threadIdx.x] = mine;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_active = cnt[0];
+}
+
 template <typename T>
 __global__ void k_fill_uniform(T *__restrict__ f, int Cp, int C, Lattice<T> k) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= Cp) return;
 #pragma unroll
   for (int q = 0; q < Q; ++q) f[(size_t)q * Cp + c] = c < C ? k.W[q] : T(0);
+}
+
+// Moments of the current pdf per device slot; batched handles read each
+// case's current buffer. Same arithmetic as Dynamics.calc_macro.
+template <typename T>
+__global__ void k_slot_macros(const T *__restrict__ f0, const T *__restrict__ f1, int Cp, int cur,
+                              int block, const int32_t *__restrict__ tile_case,
+                              const int32_t *__restrict__ case_cur, T *__restrict__ rho,
+                              T *__restrict__ ux, T *__restrict__ uy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Cp) return;
+  if (tile_case) cur = case_cur[tile_case[c / block]];
+  const T *f = cur ? f1 : f0;
+  T v[Q], r, x, y;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = f[(size_t)q * Cp + c];
+  macro(v, r, x, y);
+  rho[c] = r;
+  ux[c] = x;
+  uy[c] = y;
+}
+
+// Inverse-distance weighting at every node (utils.weighted_avg,
+// utils/utils.py:34-45, applied to all nodes rather than boundary nodes
+// only): w = 1/d, 0 where negative (padding); sum(x w) / sum(w), sums from
+// 0 in list order. Padding entries add +0 and are skipped.
+template <typename T>
+__global__ void k_node_idw(int N, int K, const int32_t *__restrict__ nc, const T *__restrict__ nd,
+                           const T *__restrict__ rho, const T *__restrict__ ux,
+                           const T *__restrict__ uy, T *__restrict__ nrho, T *__restrict__ nux,
+                           T *__restrict__ nuy) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  T ws = T(0), sr = T(0), sx = T(0), sy = T(0);
+  for (int k = 0; k < K; ++k) {
+    const int c = nc[(size_t)n * K + k];
+    T w = T(1) / nd[(size_t)n * K + k];
+    if (w < T(0)) w = T(0);
+    ws += w;
+    if (c >= 0) {
+      sr += rho[c] * w;
+      sx += ux[c] * w;
+      sy += uy[c] * w;
+    }
+  }
+  nrho[n] = sr / ws;
+  nux[n] = sx / ws;
+  nuy[n] = sy / ws;
+}
+
+// Node -> cell: inverse-distance weighting of a cell's vertices (weights
+// 1/d, d the vertex-to-centroid distance, vertices in ascending node id).
+// Slots without vertices (batch padding) get rho = 1, u = 0.
+template <typename T>
+__global__ void k_cell_from_nodes(int Cp, const int32_t *__restrict__ off,
+                                  const int32_t *__restrict__ node, const T *__restrict__ d,
+                                  const T *__restrict__ nrho, const T *__restrict__ nux,
+                                  const T *__restrict__ nuy, T *__restrict__ rho,
+                                  T *__restrict__ ux, T *__restrict__ uy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Cp) return;
+  if (off[c] == off[c + 1]) {
+    rho[c] = T(1);
+    ux[c] = T(0);
+    uy[c] = T(0);
+    return;
+  }
+  T ws = T(0), sr = T(0), sx = T(0), sy = T(0);
+  for (int e = off[c]; e < off[c + 1]; ++e) {
+    const int n = node[e];
+    const T w = T(1) / d[e];
+    ws += w;
+    sr += nrho[n] * w;
+    sx += nux[n] * w;
+    sy += nuy[n] * w;
+  }
+  rho[c] = sr / ws;
+  ux[c] = sx / ws;
+  uy[c] = sy / ws;
+}
+
+// pdf = feq(rho, u) per slot (D2Q9.calc_eq arithmetic), q-major.
+template <typename T>
+__global__ void k_equilibrium_fill(int Cp, Lattice<T> k, const T *__restrict__ rho,
+                                   const T *__restrict__ ux, const T *__restrict__ uy,
+                                   T *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Cp) return;
+  T eq[Q];
+  equilibrium(k, rho[c], ux[c], uy[c], eq);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) out[(size_t)q * Cp + c] = eq[q];
 }
 
 // ----------------------------------------------------------------------------
@@ -846,6 +1019,25 @@ struct Handle {
        *fd1 = nullptr, *fnx = nullptr, *fny = nullptr, *fL = nullptr;
   void *partials = nullptr, *resid = nullptr;
   void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
+  void *tile_count = nullptr;
+  // all-node interpolation: node -> (slot, d) lists [N][K], cell -> vertex
+  // CSR in slot order, and lazily allocated work arrays
+  void *all_nc = nullptr, *all_nd = nullptr, *cn_off = nullptr, *cn_node = nullptr, *cn_d = nullptr;
+  void *w_rho = nullptr, *w_ux = nullptr, *w_uy = nullptr, *w_nrho = nullptr, *w_nux = nullptr,
+       *w_nuy = nullptr, *w_pdf = nullptr;
+  // batched solves
+  int batch = 0, ncases = 1;
+  std::vector<int32_t> case_tile_off_host;
+  std::vector<int64_t> case_cell_off;  // reference cell range of each case
+  void *tile_case = nullptr, *case_tile_off = nullptr, *case_active = nullptr,
+       *case_cur = nullptr, *case_steps = nullptr, *case_status = nullptr,
+       *case_resid = nullptr, *case_best = nullptr, *case_invtau = nullptr,
+       *case_dt = nullptr, *n_active = nullptr;
+  std::vector<double> case_tau_host, case_dt_host;
+  hipGraphExec_t bexec = nullptr;
+  hipGraph_t bgraph = nullptr;
+  int bgraph_every = 0;
+  double bgraph_tol = 0.0, bgraph_growth = 0.0;
   int cur = 0;           // buffer holding the current pdf
   int64_t steps = 0;     // steps taken since the last init/set of the cell pdf
   bool has_prev = false; // f[cur^1] holds the pdf at the start of the last step
@@ -859,6 +1051,8 @@ struct Handle {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
+    if (bexec) (void)hipGraphExecDestroy(bexec);
+    if (bgraph) (void)hipGraphDestroy(bgraph);
     for (auto &b : bufs)
       if (b.p) (void)hipFree(b.p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -916,6 +1110,15 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.node_pdf = static_cast<const T *>(h.node_pdf);
   a.partials = static_cast<double *>(h.partials);
   a.block_tile = static_cast<const int32_t *>(h.block_tile);
+  a.tile_count = static_cast<const int32_t *>(h.tile_count);
+  a.batch = h.batch;
+  a.tile_case = static_cast<const int32_t *>(h.tile_case);
+  a.case_active = static_cast<const int32_t *>(h.case_active);
+  a.case_cur = static_cast<const int32_t *>(h.case_cur);
+  a.case_invtau = static_cast<const T *>(h.case_invtau);
+  a.case_dt = static_cast<const T *>(h.case_dt);
+  a.f0 = static_cast<const T *>(h.f[0]);
+  a.f1 = static_cast<T *>(h.f[1]);
   a.halo_off = static_cast<const int32_t *>(h.halo_off);
   a.halo_cells = static_cast<const int32_t *>(h.halo_cells);
   a.max_halo = h.max_halo;
@@ -972,8 +1175,59 @@ static int launch_step(Handle &h, int src, bool resid) {
 }
 
 static int launch_step_any(Handle &h, int src, bool resid) {
-  return h.precision == FVDBM_FP64 ? launch_step<double>(h, src, resid)
-                                   : launch_step<float>(h, src, resid);
+  int rc = h.precision == FVDBM_FP64 ? launch_step<double>(h, src, resid)
+                                     : launch_step<float>(h, src, resid);
+  if (rc || !h.batch) return rc;
+  // batched: active cases flip parity and count the step
+  hipLaunchKernelGGL(k_batch_advance, dim3((h.ncases + 255) / 256), dim3(256), 0, h.stream,
+                     h.ncases, static_cast<const int32_t *>(h.case_active),
+                     static_cast<int32_t *>(h.case_cur), static_cast<int64_t *>(h.case_steps));
+  HIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+static int launch_batch_check(Handle &h, double tol, double growth) {
+  hipLaunchKernelGGL(k_batch_check, dim3(1), dim3(256), 0, h.stream, h.ncases,
+                     static_cast<const int32_t *>(h.case_tile_off),
+                     static_cast<const double *>(h.partials), tol, growth,
+                     static_cast<int32_t *>(h.case_active), static_cast<int32_t *>(h.case_status),
+                     static_cast<double *>(h.case_resid), static_cast<double *>(h.case_best),
+                     static_cast<int32_t *>(h.n_active));
+  HIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+// One check interval of a batched run as a graph: (every - 1) plain steps,
+// then a residual step and the per-case convergence check.
+static int ensure_batch_graph(Handle &h, int every, double tol, double growth) {
+  if (h.bexec && h.bgraph_every == every && h.bgraph_tol == tol && h.bgraph_growth == growth)
+    return FVDBM_OK;
+  if (h.bexec) {
+    (void)hipGraphExecDestroy(h.bexec);
+    h.bexec = nullptr;
+  }
+  if (h.bgraph) {
+    (void)hipGraphDestroy(h.bgraph);
+    h.bgraph = nullptr;
+  }
+  HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  int rc = FVDBM_OK;
+  for (int s = 0; s < every - 1 && rc == FVDBM_OK; ++s) rc = launch_step_any(h, 0, false);
+  if (!rc) rc = launch_step_any(h, 0, true);
+  if (!rc) rc = launch_batch_check(h, tol, growth);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h.bgraph = g;
+  HIP_TRY(hipGraphInstantiate(&h.bexec, g, nullptr, nullptr, 0));
+  h.bgraph_every = every;
+  h.bgraph_tol = tol;
+  h.bgraph_growth = growth;
+  return FVDBM_OK;
 }
 
 static int ensure_graph(Handle &h, int parity) {
@@ -1000,7 +1254,7 @@ static int ensure_graph(Handle &h, int parity) {
 static int run_steps(Handle &h, int64_t n, bool resid_last) {
   if (n <= 0) return FVDBM_OK;
   int64_t plain = resid_last ? n - 1 : n;
-  if (h.graph_steps > 0) {
+  if (h.graph_steps > 0 && !h.batch) {
     while (plain >= h.graph_steps) {
       int rc = ensure_graph(h, h.cur);
       if (rc) return rc;
@@ -1027,10 +1281,12 @@ static int run_steps(Handle &h, int64_t n, bool resid_last) {
 }
 
 template <typename T>
-static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
+static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell_case = nullptr,
+                        int ncases = 1) {
   HostLayout<T> L;
   std::string err;
-  if (!build_layout<T>(m, h.reorder, h.block, L, err)) return fail(FVDBM_ERR_INVALID, err);
+  if (!build_layout<T>(m, h.reorder, h.block, L, err, cell_case, ncases))
+    return fail(FVDBM_ERR_INVALID, err);
   h.C = L.C;
   h.Cp = L.Cp;
   h.F = L.F;
@@ -1089,6 +1345,31 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
   if (!rc) rc = h.upload(&h.entry_pair_off, L.entry_pair_off);
   if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
   if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
+  if (!rc) rc = h.upload(&h.tile_count, L.tile_count);
+  if (!rc && cell_case) {
+    h.batch = 1;
+    h.ncases = L.ncases;
+    h.case_tile_off_host = L.case_tile_off;
+    rc = h.upload(&h.tile_case, L.tile_case);
+    if (!rc) rc = h.upload(&h.case_tile_off, L.case_tile_off);
+    std::vector<int32_t> ones(h.ncases, 1), zeros(h.ncases, 0);
+    std::vector<int64_t> z64(h.ncases, 0);
+    std::vector<double> zd(h.ncases, 0.0), inf(h.ncases, INFINITY);
+    std::vector<T> it(h.ncases), dtv(h.ncases);
+    for (int k = 0; k < h.ncases; ++k) {
+      it[k] = T(1.0 / h.case_tau_host[k]);  // 1/tau in double, then rounded
+      dtv[k] = T(h.case_dt_host[k]);
+    }
+    if (!rc) rc = h.upload(&h.case_active, ones);
+    if (!rc) rc = h.upload(&h.case_cur, zeros);
+    if (!rc) rc = h.upload(&h.case_steps, z64);
+    if (!rc) rc = h.upload(&h.case_status, zeros);
+    if (!rc) rc = h.upload(&h.case_resid, zd);
+    if (!rc) rc = h.upload(&h.case_best, inf);
+    if (!rc) rc = h.upload(&h.case_invtau, it);
+    if (!rc) rc = h.upload(&h.case_dt, dtv);
+    if (!rc) rc = h.alloc(&h.n_active, sizeof(int32_t));
+  }
   if (!rc) rc = h.upload(&h.halo_off, L.halo_off);
   if (!rc) rc = h.upload(&h.halo_cells, L.halo_cells);
   if (!rc) {
@@ -1131,6 +1412,36 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
   if (!rc) rc = h.upload(&h.fL, L.fL);
   if (!rc) rc = h.alloc(&h.partials, (size_t)2 * h.ntiles * sizeof(double));
   if (!rc) rc = h.alloc(&h.resid, 2 * sizeof(double));
+  if (!rc) {
+    // node lists with cells as device slots (all nodes, for the all-node
+    // interpolation), and each cell's vertices in ascending node id
+    const int K = m.k_max;
+    const T *nd = static_cast<const T *>(m.node_dists);
+    std::vector<int32_t> nc((size_t)h.N * K);
+    std::vector<T> ndv((size_t)h.N * K);
+    std::vector<int32_t> cnt((size_t)h.Cp + 1, 0);
+    for (size_t i = 0; i < nc.size(); ++i) {
+      const int c = m.node_cells[i];
+      nc[i] = c >= 0 ? L.old_to_new[c] : -1;
+      ndv[i] = nd[i];
+      if (c >= 0) cnt[(size_t)nc[i] + 1] += 1;
+    }
+    for (int c = 0; c < h.Cp; ++c) cnt[c + 1] += cnt[c];
+    std::vector<int32_t> cn_node(cnt[h.Cp]), fill(cnt.begin(), cnt.end() - 1);
+    std::vector<T> cn_d(cnt[h.Cp]);
+    for (int n = 0; n < h.N; ++n)
+      for (int k = 0; k < K; ++k) {
+        const int s = nc[(size_t)n * K + k];
+        if (s < 0) continue;
+        cn_node[fill[s]] = n;
+        cn_d[fill[s]++] = ndv[(size_t)n * K + k];
+      }
+    rc = h.upload(&h.all_nc, nc);
+    if (!rc) rc = h.upload(&h.all_nd, ndv);
+    if (!rc) rc = h.upload(&h.cn_off, cnt);
+    if (!rc) rc = h.upload(&h.cn_node, cn_node);
+    if (!rc) rc = h.upload(&h.cn_d, cn_d);
+  }
   if (rc) return rc;
   // node rho/vel hold the boundary-condition values (reference node order)
   std::vector<T> rho(h.N), vel(2 * (size_t)h.N);
@@ -1149,13 +1460,24 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h) {
 template <typename T>
 static int init_uniform_typed(Handle &h) {
   Lattice<T> k = make_lattice<T>(h.tau, h.dt);
+  // every slot, holes included (cells of later cases sit past slot C)
   for (int i = 0; i < 2; ++i)
     hipLaunchKernelGGL((k_fill_uniform<T>), dim3((h.Cp + 255) / 256), dim3(256), 0, h.stream,
-                       static_cast<T *>(h.f[i]), h.Cp, h.C, k);
+                       static_cast<T *>(h.f[i]), h.Cp, h.Cp, k);
   // node pdf <- W (both type 0 and active nodes)
   hipLaunchKernelGGL((k_fill_uniform<T>), dim3((h.N + 255) / 256), dim3(256), 0, h.stream,
                      static_cast<T *>(h.node_pdf), h.N, h.N, k);
   HIP_TRY(hipGetLastError());
+  if (h.batch) {
+    std::vector<int32_t> ones(h.ncases, 1), zeros(h.ncases, 0);
+    std::vector<int64_t> z64(h.ncases, 0);
+    std::vector<double> inf(h.ncases, INFINITY);
+    HIP_TRY(hipMemcpy(h.case_active, ones.data(), 4 * (size_t)h.ncases, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.case_cur, zeros.data(), 4 * (size_t)h.ncases, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.case_status, zeros.data(), 4 * (size_t)h.ncases, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.case_steps, z64.data(), 8 * (size_t)h.ncases, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.case_best, inf.data(), 8 * (size_t)h.ncases, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipStreamSynchronize(h.stream));
   h.cur = 0;
   h.steps = 0;
@@ -1207,6 +1529,8 @@ static int observe_typed(Handle &h) {
 
 template <typename T>
 static int get_typed(Handle &h, int field, void *dst) {
+  if (h.batch && field != FVDBM_CELL_PDF)
+    return fail(FVDBM_ERR_INVALID, "a batched handle exposes FVDBM_CELL_PDF only");
   T *out = static_cast<T *>(dst);
   const size_t es = sizeof(T);
   HIP_TRY(hipStreamSynchronize(h.stream));
@@ -1214,8 +1538,9 @@ static int get_typed(Handle &h, int field, void *dst) {
     std::vector<T> tmp((size_t)comps * stride);
     hipError_t e = hipMemcpy(tmp.data(), dev, tmp.size() * es, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail(FVDBM_ERR_HIP, hipGetErrorString(e));
-    for (int i = 0; i < h.C; ++i) {
+    for (int i = 0; i < h.Cp; ++i) {
       const int o = h.new_to_old[i];
+      if (o < 0) continue;
       for (int q = 0; q < comps; ++q) out[(size_t)o * comps + q] = tmp[(size_t)q * stride + i];
     }
     return FVDBM_OK;
@@ -1230,6 +1555,23 @@ static int get_typed(Handle &h, int field, void *dst) {
   };
   switch (field) {
     case FVDBM_CELL_PDF:
+      if (h.batch) {  // each case's current buffer
+        std::vector<int32_t> cur(h.ncases);
+        HIP_TRY(hipMemcpy(cur.data(), h.case_cur, cur.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<T> t0((size_t)Q * h.Cp), t1((size_t)Q * h.Cp);
+        HIP_TRY(hipMemcpy(t0.data(), h.f[0], t0.size() * es, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(t1.data(), h.f[1], t1.size() * es, hipMemcpyDeviceToHost));
+        for (int k = 0; k < h.ncases; ++k) {
+          const std::vector<T> &src = cur[k] ? t1 : t0;
+          for (int i = h.case_tile_off_host[k] * h.block; i < h.case_tile_off_host[k + 1] * h.block;
+               ++i) {
+            const int o = h.new_to_old[i];
+            if (o < 0) continue;
+            for (int q = 0; q < Q; ++q) out[(size_t)o * Q + q] = src[(size_t)q * h.Cp + i];
+          }
+        }
+        return FVDBM_OK;
+      }
       return cell_q_major(h.f[h.cur], Q, h.Cp);
     case FVDBM_CELL_RHO:
     case FVDBM_CELL_VEL:
@@ -1280,14 +1622,29 @@ static int get_typed(Handle &h, int field, void *dst) {
 
 template <typename T>
 static int set_typed(Handle &h, int field, const void *src) {
+  if (h.batch && field != FVDBM_CELL_PDF)
+    return fail(FVDBM_ERR_INVALID, "a batched handle exposes FVDBM_CELL_PDF only");
   const T *in = static_cast<const T *>(src);
   const size_t es = sizeof(T);
   HIP_TRY(hipStreamSynchronize(h.stream));
   if (field == FVDBM_CELL_PDF) {
     std::vector<T> tmp((size_t)Q * h.Cp, T(0));
-    for (int i = 0; i < h.C; ++i) {
+    for (int i = 0; i < h.Cp; ++i) {
       const int o = h.new_to_old[i];
+      if (o < 0) continue;
       for (int q = 0; q < Q; ++q) tmp[(size_t)q * h.Cp + i] = in[(size_t)o * Q + q];
+    }
+    if (h.batch) {  // into each case's current buffer
+      std::vector<int32_t> cur(h.ncases);
+      HIP_TRY(hipMemcpy(cur.data(), h.case_cur, cur.size() * 4, hipMemcpyDeviceToHost));
+      for (int k = 0; k < h.ncases; ++k) {
+        const size_t i0 = (size_t)h.case_tile_off_host[k] * h.block;
+        const size_t n = (size_t)(h.case_tile_off_host[k + 1] - h.case_tile_off_host[k]) * h.block;
+        for (int q = 0; q < Q; ++q)
+          HIP_TRY(hipMemcpy(static_cast<T *>(h.f[cur[k]]) + (size_t)q * h.Cp + i0,
+                            tmp.data() + (size_t)q * h.Cp + i0, n * es, hipMemcpyHostToDevice));
+      }
+      return FVDBM_OK;
     }
     // write the current buffer only: f[cur^1] keeps the pdf at the start of
     // the last step, which the derived cell fields still describe
@@ -1327,6 +1684,157 @@ static int set_typed(Handle &h, int field, const void *src) {
   return fail(FVDBM_ERR_INVALID, "field cannot be set");
 }
 
+// ----------------------------------------------------------------------------
+// all-node interpolation and equilibrium initialisation
+// ----------------------------------------------------------------------------
+template <typename T>
+static int ensure_work(Handle &h) {
+  if (h.w_pdf) return FVDBM_OK;
+  const size_t es = sizeof(T);
+  int rc = h.alloc(&h.w_rho, (size_t)h.Cp * es);
+  if (!rc) rc = h.alloc(&h.w_ux, (size_t)h.Cp * es);
+  if (!rc) rc = h.alloc(&h.w_uy, (size_t)h.Cp * es);
+  if (!rc) rc = h.alloc(&h.w_nrho, (size_t)h.N * es);
+  if (!rc) rc = h.alloc(&h.w_nux, (size_t)h.N * es);
+  if (!rc) rc = h.alloc(&h.w_nuy, (size_t)h.N * es);
+  if (!rc) rc = h.alloc(&h.w_pdf, (size_t)Q * h.Cp * es);
+  return rc;
+}
+
+template <typename T>
+static int launch_slot_macros(Handle &h) {
+  hipLaunchKernelGGL((k_slot_macros<T>), dim3((h.Cp + 255) / 256), dim3(256), 0, h.stream,
+                     static_cast<const T *>(h.f[0]), static_cast<const T *>(h.f[1]), h.Cp, h.cur,
+                     h.block, h.batch ? static_cast<const int32_t *>(h.tile_case) : nullptr,
+                     static_cast<const int32_t *>(h.case_cur), static_cast<T *>(h.w_rho),
+                     static_cast<T *>(h.w_ux), static_cast<T *>(h.w_uy));
+  HIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+// rho [N] and u [N][2] at every node from the current cell pdf
+template <typename T>
+static int node_fields_typed(Handle &h, void *rho_out, void *vel_out) {
+  int rc = ensure_work<T>(h);
+  if (!rc) rc = launch_slot_macros<T>(h);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_node_idw<T>), dim3((h.N + 255) / 256), dim3(256), 0, h.stream, h.N, h.K,
+                     static_cast<const int32_t *>(h.all_nc), static_cast<const T *>(h.all_nd),
+                     static_cast<const T *>(h.w_rho), static_cast<const T *>(h.w_ux),
+                     static_cast<const T *>(h.w_uy), static_cast<T *>(h.w_nrho),
+                     static_cast<T *>(h.w_nux), static_cast<T *>(h.w_nuy));
+  HIP_TRY(hipGetLastError());
+  std::vector<T> r(h.N), x(h.N), y(h.N);
+  HIP_TRY(hipMemcpyAsync(r.data(), h.w_nrho, h.N * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipMemcpyAsync(x.data(), h.w_nux, h.N * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipMemcpyAsync(y.data(), h.w_nuy, h.N * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipStreamSynchronize(h.stream));
+  T *ro = static_cast<T *>(rho_out), *vo = static_cast<T *>(vel_out);
+  for (int n = 0; n < h.N; ++n) {
+    if (ro) ro[n] = r[n];
+    if (vo) {
+      vo[2 * (size_t)n] = x[n];
+      vo[2 * (size_t)n + 1] = y[n];
+    }
+  }
+  return FVDBM_OK;
+}
+
+// rho [C] and u [C][2] of the current cell pdf (reference cell order)
+template <typename T>
+static int cell_fields_typed(Handle &h, void *rho_out, void *vel_out) {
+  int rc = ensure_work<T>(h);
+  if (!rc) rc = launch_slot_macros<T>(h);
+  if (rc) return rc;
+  std::vector<T> r(h.Cp), x(h.Cp), y(h.Cp);
+  HIP_TRY(hipMemcpyAsync(r.data(), h.w_rho, h.Cp * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipMemcpyAsync(x.data(), h.w_ux, h.Cp * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipMemcpyAsync(y.data(), h.w_uy, h.Cp * sizeof(T), hipMemcpyDeviceToHost, h.stream));
+  HIP_TRY(hipStreamSynchronize(h.stream));
+  T *ro = static_cast<T *>(rho_out), *vo = static_cast<T *>(vel_out);
+  for (int i = 0; i < h.Cp; ++i) {
+    const int o = h.new_to_old[i];
+    if (o < 0) continue;
+    if (ro) ro[o] = r[i];
+    if (vo) {
+      vo[2 * (size_t)o] = x[i];
+      vo[2 * (size_t)o + 1] = y[i];
+    }
+  }
+  return FVDBM_OK;
+}
+
+// The q-major pdf in w_pdf becomes the current cell pdf: of the handle, or
+// of every case of a batch (into each case's current buffer).
+template <typename T>
+static int install_work_pdf(Handle &h) {
+  const size_t es = sizeof(T);
+  if (!h.batch) {
+    HIP_TRY(hipMemcpyAsync(h.f[h.cur], h.w_pdf, (size_t)Q * h.Cp * es, hipMemcpyDeviceToDevice,
+                           h.stream));
+  } else {
+    std::vector<int32_t> cur(h.ncases);
+    HIP_TRY(hipMemcpyAsync(cur.data(), h.case_cur, cur.size() * 4, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    for (int k = 0; k < h.ncases; ++k) {
+      const size_t i0 = (size_t)h.case_tile_off_host[k] * h.block;
+      const size_t n = (size_t)(h.case_tile_off_host[k + 1] - h.case_tile_off_host[k]) * h.block;
+      for (int q = 0; q < Q; ++q)
+        HIP_TRY(hipMemcpyAsync(static_cast<T *>(h.f[cur[k]]) + (size_t)q * h.Cp + i0,
+                               static_cast<const T *>(h.w_pdf) + (size_t)q * h.Cp + i0, n * es,
+                               hipMemcpyDeviceToDevice, h.stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(h.stream));
+  h.observed = false;
+  return FVDBM_OK;
+}
+
+// cell pdf <- feq(rho, u): from cell values (reference cell order) or, with
+// at_nodes, from node values interpolated to the cells
+template <typename T>
+static int set_equilibrium_typed(Handle &h, const void *rho_in, const void *vel_in, bool at_nodes) {
+  int rc = ensure_work<T>(h);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(h.stream));
+  const T *ri = static_cast<const T *>(rho_in), *vi = static_cast<const T *>(vel_in);
+  if (!at_nodes) {
+    std::vector<T> r(h.Cp, T(1)), x(h.Cp, T(0)), y(h.Cp, T(0));
+    for (int i = 0; i < h.Cp; ++i) {
+      const int o = h.new_to_old[i];
+      if (o < 0) continue;
+      r[i] = ri[o];
+      x[i] = vi[2 * (size_t)o];
+      y[i] = vi[2 * (size_t)o + 1];
+    }
+    HIP_TRY(hipMemcpy(h.w_rho, r.data(), r.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.w_ux, x.data(), x.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.w_uy, y.data(), y.size() * sizeof(T), hipMemcpyHostToDevice));
+  } else {
+    std::vector<T> x(h.N), y(h.N);
+    for (int n = 0; n < h.N; ++n) {
+      x[n] = vi[2 * (size_t)n];
+      y[n] = vi[2 * (size_t)n + 1];
+    }
+    HIP_TRY(hipMemcpy(h.w_nrho, ri, h.N * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.w_nux, x.data(), x.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h.w_nuy, y.data(), y.size() * sizeof(T), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((k_cell_from_nodes<T>), dim3((h.Cp + 255) / 256), dim3(256), 0, h.stream,
+                       h.Cp, static_cast<const int32_t *>(h.cn_off),
+                       static_cast<const int32_t *>(h.cn_node), static_cast<const T *>(h.cn_d),
+                       static_cast<const T *>(h.w_nrho), static_cast<const T *>(h.w_nux),
+                       static_cast<const T *>(h.w_nuy), static_cast<T *>(h.w_rho),
+                       static_cast<T *>(h.w_ux), static_cast<T *>(h.w_uy));
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL((k_equilibrium_fill<T>), dim3((h.Cp + 255) / 256), dim3(256), 0, h.stream,
+                     h.Cp, make_lattice<T>(h.tau, h.dt), static_cast<const T *>(h.w_rho),
+                     static_cast<const T *>(h.w_ux), static_cast<const T *>(h.w_uy),
+                     static_cast<T *>(h.w_pdf));
+  HIP_TRY(hipGetLastError());
+  return install_work_pdf<T>(h);
+}
+
 }  // namespace fvdbm
 
 // ============================================================================
@@ -1355,7 +1863,8 @@ int fvdbm_variant_count(int32_t *count) {
   return FVDBM_OK;
 }
 
-int fvdbm_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, void **out) {
+static int create_impl(const fvdbm_mesh_desc *mesh, const fvdbm_params *p,
+                       const fvdbm_batch_desc *batch, void **out) {
   if (!mesh || !p || !out) return fail(FVDBM_ERR_INVALID, "null argument");
   *out = nullptr;
   if (p->precision != FVDBM_FP32 && p->precision != FVDBM_FP64)
@@ -1382,16 +1891,106 @@ int fvdbm_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, void **out)
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(FVDBM_ERR_HIP, "no HIP device available");
+  const int32_t *cell_case = nullptr;
+  int ncases = 1;
+  if (batch) {
+    if (batch->n_cases < 1 || !batch->cell_case || !batch->case_tau || !batch->case_dt)
+      return fail(FVDBM_ERR_INVALID, "batch descriptor needs n_cases >= 1 and its arrays");
+    ncases = batch->n_cases;
+    cell_case = batch->cell_case;
+    for (int k = 0; k < ncases; ++k) {
+      if (!(batch->case_tau[k] > 0.0) || !std::isfinite(batch->case_dt[k]))
+        return fail(FVDBM_ERR_INVALID, "case tau must be > 0 and dt finite");
+      h->case_tau_host.push_back(batch->case_tau[k]);
+      h->case_dt_host.push_back(batch->case_dt[k]);
+    }
+    h->graph_steps = -1;
+  }
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
   HIP_TRY(hipEventCreate(&h->ev1));
-  int rc = p->precision == FVDBM_FP64 ? fvdbm::create_typed<double>(*mesh, *h)
-                                      : fvdbm::create_typed<float>(*mesh, *h);
+  int rc = p->precision == FVDBM_FP64
+               ? fvdbm::create_typed<double>(*mesh, *h, cell_case, ncases)
+               : fvdbm::create_typed<float>(*mesh, *h, cell_case, ncases);
   if (rc) return rc;
   rc = p->precision == FVDBM_FP64 ? fvdbm::init_uniform_typed<double>(*h)
                                   : fvdbm::init_uniform_typed<float>(*h);
   if (rc) return rc;
   *out = h.release();
+  return FVDBM_OK;
+}
+
+int fvdbm_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, void **out) {
+  return create_impl(mesh, p, nullptr, out);
+}
+
+int fvdbm_create_batch(const fvdbm_mesh_desc *mesh, const fvdbm_params *p,
+                       const fvdbm_batch_desc *batch, void **out) {
+  if (!batch) return fail(FVDBM_ERR_INVALID, "null batch descriptor");
+  return create_impl(mesh, p, batch, out);
+}
+
+int fvdbm_batch_run(void *handle, int64_t max_steps, int32_t check_every, double tol,
+                    double growth, int32_t *n_active, int64_t *steps_run) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !n_active || !steps_run) return fail(FVDBM_ERR_INVALID, "null argument");
+  if (!h->batch) return fail(FVDBM_ERR_STATE, "not a batched handle");
+  if (check_every < 1 || max_steps < 0) return fail(FVDBM_ERR_INVALID, "bad check interval");
+  int rc = fvdbm::ensure_batch_graph(*h, check_every, tol, growth);
+  if (rc) return rc;
+  int32_t na = h->ncases;
+  int64_t done = 0;
+  while (done < max_steps && na > 0) {
+    HIP_TRY(hipGraphLaunch(h->bexec, h->stream));
+    done += check_every;
+    HIP_TRY(hipMemcpyAsync(&na, h->n_active, sizeof(na), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  *n_active = na;
+  *steps_run = done;
+  return FVDBM_OK;
+}
+
+int fvdbm_batch_status(void *handle, int64_t *steps, double *residual, int32_t *status,
+                       int32_t *active) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  if (!h->batch) return fail(FVDBM_ERR_STATE, "not a batched handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t n = h->ncases;
+  if (steps) HIP_TRY(hipMemcpy(steps, h->case_steps, 8 * n, hipMemcpyDeviceToHost));
+  if (residual) HIP_TRY(hipMemcpy(residual, h->case_resid, 8 * n, hipMemcpyDeviceToHost));
+  if (status) HIP_TRY(hipMemcpy(status, h->case_status, 4 * n, hipMemcpyDeviceToHost));
+  if (active) HIP_TRY(hipMemcpy(active, h->case_active, 4 * n, hipMemcpyDeviceToHost));
+  return FVDBM_OK;
+}
+
+int fvdbm_batch_set_active(void *handle, const int32_t *active, int32_t reset_steps) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !active) return fail(FVDBM_ERR_INVALID, "null argument");
+  if (!h->batch) return fail(FVDBM_ERR_STATE, "not a batched handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t n = h->ncases;
+  std::vector<int32_t> act(active, active + n), st(n);
+  std::vector<double> best(n);
+  std::vector<int64_t> steps(n);
+  HIP_TRY(hipMemcpy(st.data(), h->case_status, 4 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(best.data(), h->case_best, 8 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(steps.data(), h->case_steps, 8 * n, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < n; ++k) {
+    act[k] = act[k] ? 1 : 0;
+    if (act[k]) {  // (re)started: running, no residual history
+      st[k] = fvdbm::kRunning;
+      best[k] = INFINITY;
+      if (reset_steps) steps[k] = 0;
+    } else if (st[k] == fvdbm::kRunning) {
+      st[k] = fvdbm::kStopped;
+    }
+  }
+  HIP_TRY(hipMemcpy(h->case_active, act.data(), 4 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->case_status, st.data(), 4 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->case_best, best.data(), 8 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->case_steps, steps.data(), 8 * n, hipMemcpyHostToDevice));
   return FVDBM_OK;
 }
 
@@ -1419,6 +2018,7 @@ int fvdbm_info_get(void *handle, fvdbm_info *o) {
   o->graph_steps = h->graph_steps;
   o->steps_done = h->steps;
   o->device_bytes = h->device_bytes;
+  o->n_cases = h->ncases;
   return FVDBM_OK;
 }
 
@@ -1427,6 +2027,36 @@ int fvdbm_init_uniform(void *handle) {
   if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
   return h->precision == FVDBM_FP64 ? fvdbm::init_uniform_typed<double>(*h)
                                     : fvdbm::init_uniform_typed<float>(*h);
+}
+
+int fvdbm_node_fields(void *handle, void *node_rho, void *node_vel) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || (!node_rho && !node_vel)) return fail(FVDBM_ERR_INVALID, "null argument");
+  return h->precision == FVDBM_FP64 ? fvdbm::node_fields_typed<double>(*h, node_rho, node_vel)
+                                    : fvdbm::node_fields_typed<float>(*h, node_rho, node_vel);
+}
+
+int fvdbm_cell_fields(void *handle, void *cell_rho, void *cell_vel) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || (!cell_rho && !cell_vel)) return fail(FVDBM_ERR_INVALID, "null argument");
+  return h->precision == FVDBM_FP64 ? fvdbm::cell_fields_typed<double>(*h, cell_rho, cell_vel)
+                                    : fvdbm::cell_fields_typed<float>(*h, cell_rho, cell_vel);
+}
+
+int fvdbm_set_equilibrium(void *handle, const void *cell_rho, const void *cell_vel) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !cell_rho || !cell_vel) return fail(FVDBM_ERR_INVALID, "null argument");
+  return h->precision == FVDBM_FP64
+             ? fvdbm::set_equilibrium_typed<double>(*h, cell_rho, cell_vel, false)
+             : fvdbm::set_equilibrium_typed<float>(*h, cell_rho, cell_vel, false);
+}
+
+int fvdbm_set_equilibrium_nodes(void *handle, const void *node_rho, const void *node_vel) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !node_rho || !node_vel) return fail(FVDBM_ERR_INVALID, "null argument");
+  return h->precision == FVDBM_FP64
+             ? fvdbm::set_equilibrium_typed<double>(*h, node_rho, node_vel, true)
+             : fvdbm::set_equilibrium_typed<float>(*h, node_rho, node_vel, true);
 }
 
 int fvdbm_set_field(void *handle, int32_t field, const void *src) {
@@ -1537,7 +2167,9 @@ int fvdbm_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy) {
 int fvdbm_get_permutation(void *handle, int32_t *new_to_old) {
   Handle *h = static_cast<Handle *>(handle);
   if (!h || !new_to_old) return fail(FVDBM_ERR_INVALID, "null argument");
-  std::copy(h->new_to_old.begin(), h->new_to_old.end(), new_to_old);
+  size_t j = 0;
+  for (int32_t o : h->new_to_old)
+    if (o >= 0) new_to_old[j++] = o;
   return FVDBM_OK;
 }
 

@@ -132,7 +132,7 @@ std::vector<int32_t> reverse_cuthill_mckee(int n, const std::vector<int32_t> &xa
 
 template <typename T>
 bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
-                  HostLayout<T> &L, std::string &err) {
+                  HostLayout<T> &L, std::string &err, const int32_t *cell_case, int ncases) {
   const int C = m.n_cells, F = m.n_faces, N = m.n_nodes, K = m.k_max;
   if (C <= 0 || F <= 0 || N <= 0 || K < 0) {
     err = "mesh must have cells, faces and nodes";
@@ -236,15 +236,54 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
                             [&](int32_t c) { return has_ghost[c] != 0; });
     }
   }
-  L.new_to_old = perm;
+  const int pad = tile > 0 ? tile : 1;
+  // device slots: cases (if any) grouped and tile-aligned, holes = -1
+  if (cell_case) {
+    if (ncases < 1) {
+      err = "ncases must be >= 1";
+      return false;
+    }
+    for (int c = 0; c < C; ++c)
+      if (cell_case[c] < 0 || cell_case[c] >= ncases) {
+        err = "cell case id out of range at cell " + std::to_string(c);
+        return false;
+      }
+    for (int c = 0; c < C; ++c)
+      for (int k = 0; k < 3; ++k)
+        if (other[3 * c + k] >= 0 && cell_case[other[3 * c + k]] != cell_case[c]) {
+          err = "a face joins cells of different cases";
+          return false;
+        }
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int32_t a, int32_t b) { return cell_case[a] < cell_case[b]; });
+  }
+  L.ncases = cell_case ? ncases : 1;
+  L.new_to_old.clear();
+  L.case_tile_off.assign(L.ncases + 1, 0);
+  {
+    size_t i = 0;
+    for (int k = 0; k < L.ncases; ++k) {
+      L.case_tile_off[k] = (int32_t)(L.new_to_old.size() / pad);
+      while (i < perm.size() && (!cell_case || cell_case[perm[i]] == k))
+        L.new_to_old.push_back(perm[i++]);
+      while (L.new_to_old.size() % pad) L.new_to_old.push_back(-1);
+    }
+    L.case_tile_off[L.ncases] = (int32_t)(L.new_to_old.size() / pad);
+  }
+  L.Cp = (int)L.new_to_old.size();
+  const size_t Cp = (size_t)L.Cp;
   L.old_to_new.assign(C, 0);
-  for (int i = 0; i < C; ++i) L.old_to_new[perm[i]] = i;
+  for (size_t i = 0; i < Cp; ++i)
+    if (L.new_to_old[i] >= 0) L.old_to_new[L.new_to_old[i]] = (int32_t)i;
   L.n_bcells = 0;
   for (int c = 0; c < C; ++c) L.n_bcells += has_ghost[c];
   L.last_cell_new = L.old_to_new[C - 1];
-  const int pad = tile > 0 ? tile : 1;
-  L.Cp = ((C + pad - 1) / pad) * pad;
-  const size_t Cp = (size_t)L.Cp;
+  L.tile_count.assign(Cp / pad, 0);
+  L.tile_case.assign(Cp / pad, 0);
+  for (size_t i = 0; i < Cp; ++i)
+    if (L.new_to_old[i] >= 0) ++L.tile_count[i / pad];
+  for (int k = 0; k < L.ncases; ++k)
+    for (int t = L.case_tile_off[k]; t < L.case_tile_off[k + 1]; ++t) L.tile_case[t] = k;
 
   // --- active nodes
   std::vector<int32_t> node_an(N, -1);
@@ -313,7 +352,7 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
     return slot_of[node];
   };
   int bw = 0;
-  for (int i = 0; i < C; ++i) {
+  for (int i = 0; i < (int)Cp; ++i) {
     const int t = i / pad;
     while (cur_tile < t) {  // close the previous tile's node table
       for (int nd_ : touched) slot_of[nd_] = -1;
@@ -321,7 +360,8 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
       ++cur_tile;
       L.tile_node_off[cur_tile] = (int32_t)L.tile_nodes.size();
     }
-    int c = perm[i];
+    const int c = L.new_to_old[i];
+    if (c < 0) continue;  // hole
     for (int k = 0; k < 3; ++k) {
       int fid = m.cell_faces[3 * c + k];
       int sl = slot[3 * c + k], o = other[3 * c + k];
@@ -395,7 +435,7 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
     std::vector<int32_t> h;
     for (int t = 0; t < L.ntiles; ++t) {
       L.halo_off[t] = (int32_t)L.halo_cells.size();
-      const int lo = t * pad, hi = std::min((t + 1) * pad, C);
+      const int lo = t * pad, hi = t * pad + L.tile_count[t];
       h.clear();
       for (int k = 0; k < 3; ++k)
         for (int i = lo; i < hi; ++i) {
@@ -468,9 +508,9 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
 }
 
 template bool build_layout<float>(const fvdbm_mesh_desc &, int, int, HostLayout<float> &,
-                                  std::string &);
+                                  std::string &, const int32_t *, int);
 template bool build_layout<double>(const fvdbm_mesh_desc &, int, int, HostLayout<double> &,
-                                   std::string &);
+                                   std::string &, const int32_t *, int);
 
 }  // namespace fvdbm
 

@@ -22,7 +22,12 @@ constexpr int32_t kIndexMask = 0x3fffffff;
 
 template <typename T>
 struct HostLayout {
+  // C cells occupy Cp device slots; a slot whose new_to_old is -1 is a hole
+  // (batched layouts start every case on a tile boundary). The live cells of
+  // a tile are its first tile_count[t] slots.
   int C = 0, Cp = 0, F = 0, N = 0, K = 0;
+  int ncases = 1;
+  std::vector<int32_t> tile_count, tile_case, case_tile_off;
   int n_bcells = 0;   // cells with >= 1 ghost face, stored first when reorder==1
   int nbf = 0;        // boundary-face records
   int nan = 0;        // active (type 1 / 2) nodes
@@ -76,10 +81,12 @@ struct HostLayout {
 };
 
 // `tile` is the number of cells per thread block; the cell count is padded
-// to a multiple of it.
+// to a multiple of it. With cell_case (case id of every cell, ncases cases)
+// the cells are grouped by case and each case starts on a tile boundary.
 template <typename T>
 bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
-                  HostLayout<T> &out, std::string &err);
+                  HostLayout<T> &out, std::string &err,
+                  const int32_t *cell_case = nullptr, int ncases = 1);
 
 // Cells sorted by the Hilbert index of their centroids ([n][2]).
 std::vector<int32_t> hilbert_order(int n, const double *xy);

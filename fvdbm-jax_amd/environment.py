@@ -44,7 +44,7 @@ def _f64_or(obj, name, fallback):
     return np.asarray(fallback, np.float64) if v is None else np.asarray(v, np.float64)
 
 
-class Environment:
+class Environment(_lib.FieldsMixin):
     """Cells, faces and nodes of one mesh plus the device state that steps them."""
 
     dynamics = None  # class attribute used by create(), as in the reference
@@ -151,6 +151,9 @@ class Environment:
             self.close()
         except Exception:
             pass
+
+    def _counts(self):
+        return self.cells.size, self.nodes.size
 
     def _handle(self):
         if not self._h:

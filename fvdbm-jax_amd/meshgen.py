@@ -151,22 +151,24 @@ def cylinder_channel(target_cells=1_000_000, seed=0):
     return mesh
 
 
-def porous_channel(seed, target_cells=None, n_obstacles=None):
-    """Porous channel [0,1]x[0,0.5] with 5-40 non-overlapping discs,
-    r ~ U[0.02,0.08], centres uniform (rejection sampling with a gap of
-    0.01 between discs and from the walls). ``target_cells`` defaults to
-    U[20k,200k]."""
+def porous_channel(seed, target_cells=None, n_obstacles=None, r_range=(0.02, 0.08), gap=0.01,
+                   count_range=(5, 40), cells_range=(20_000, 200_000)):
+    """Porous channel [0,1]x[0,0.5] with non-overlapping discs, radius
+    U[r_range], centres uniform over x in [0.1+r, 0.9-r] (rejection sampling
+    with ``gap`` between discs and from the walls). The disc count defaults
+    to U{count_range} and ``target_cells`` to U[cells_range]; the defaults
+    are the 256-case batched workload (5-40 discs, r in [0.02,0.08],
+    20k-200k cells)."""
     rng = np.random.default_rng(seed)
     if n_obstacles is None:
-        n_obstacles = int(rng.integers(5, 41))
+        n_obstacles = int(rng.integers(count_range[0], count_range[1] + 1))
     if target_cells is None:
-        target_cells = int(rng.integers(20_000, 200_001))
+        target_cells = int(rng.integers(cells_range[0], cells_range[1] + 1))
     discs = []
-    gap = 0.01
     tries = 0
     while len(discs) < n_obstacles and tries < 20000:
         tries += 1
-        r = rng.uniform(0.02, 0.08)
+        r = rng.uniform(*r_range)
         cx = rng.uniform(0.1 + r, 0.9 - r)
         cy = rng.uniform(r + gap, 0.5 - r - gap)
         if all(math.hypot(cx - x, cy - y) > r + s + gap for x, y, s in discs):

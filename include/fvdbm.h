@@ -105,7 +105,25 @@ typedef struct fvdbm_info {
   int32_t precision, scheme, variant, block, graph_steps;
   int64_t steps_done;
   int64_t device_bytes;     /* device memory owned by the handle */
+  int32_t n_cases;          /* 1, or the case count of a batched handle */
 } fvdbm_info;
+
+/* Batched solves (no counterpart in the reference, which runs one mesh per
+ * Environment): the mesh descriptor holds several independent meshes
+ * concatenated (no face joins two cases), cell_case gives each cell's case,
+ * cases contiguous and in order. Each case has its own tau and delta_t, its
+ * own step count and a convergence mask; one kernel launch steps all active
+ * cases. */
+typedef struct fvdbm_batch_desc {
+  int32_t n_cases;
+  const int32_t *cell_case; /* [C] case id of each cell, nondecreasing */
+  const double *case_tau;   /* [n_cases] */
+  const double *case_dt;    /* [n_cases] */
+} fvdbm_batch_desc;
+
+/* Per-case status values of fvdbm_batch_status. */
+enum { FVDBM_RUNNING = 0, FVDBM_CONVERGED = 1, FVDBM_DIVERGED = 2, FVDBM_GROWING = 3,
+       FVDBM_STOPPED = 4 };
 
 const char *fvdbm_last_error(void);
 int fvdbm_abi_version(void);
@@ -143,6 +161,48 @@ int fvdbm_synchronize(void *handle);
  * (k_copy_probe) and returns milliseconds per copy. Used to scale the
  * FETCH_SIZE / WRITE_SIZE counters and as the attainable-bandwidth check. */
 int fvdbm_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy);
+
+/* Batched handle (see fvdbm_batch_desc). Only FVDBM_CELL_PDF can be read
+ * or written (each case's current buffer); every case starts at W. */
+int fvdbm_create_batch(const fvdbm_mesh_desc *mesh, const fvdbm_params *params,
+                       const fvdbm_batch_desc *batch, void **handle);
+/* Run active cases in intervals of check_every steps (one captured graph
+ * per interval) until none is active or max_steps have run. After each
+ * interval's last step a case stops as CONVERGED when its residual
+ * ||u^{n+1}-u^n||/||u^{n+1}|| is below tol, DIVERGED when it is not finite,
+ * GROWING when growth > 0 and it exceeds growth times its smallest residual
+ * so far. Returns the number of active cases and the steps run. */
+int fvdbm_batch_run(void *handle, int64_t max_steps, int32_t check_every, double tol,
+                    double growth, int32_t *n_active, int64_t *steps_run);
+/* Per-case steps taken, last residual, status (FVDBM_RUNNING ...), active
+ * flag; any pointer may be NULL. */
+int fvdbm_batch_status(void *handle, int64_t *steps, double *residual, int32_t *status,
+                       int32_t *active);
+/* Set the active mask; a case switched on restarts its residual history
+ * (and its step count when reset_steps). */
+int fvdbm_batch_set_active(void *handle, const int32_t *active, int32_t reset_steps);
+
+/* Interpolation and equilibrium initialisation (single and batched
+ * handles; arrays in reference order, batches concatenated case by case,
+ * element type per the handle's precision).
+ *
+ * fvdbm_node_fields: rho [N] and u [N][2] at every node, the reference's
+ * inverse-distance weighting (utils.weighted_avg, utils/utils.py:34-45; used
+ * there for boundary nodes only, Nodes.calc_density / calc_velocity,
+ * src/containers.py:364-374) of the moments of the current cell pdf.
+ * Either output may be NULL. */
+int fvdbm_node_fields(void *handle, void *node_rho, void *node_vel);
+/* rho [C] and u [C][2] of the current cell pdf (Dynamics.calc_macro,
+ * src/dynamics.py:35-47); either output may be NULL. */
+int fvdbm_cell_fields(void *handle, void *cell_rho, void *cell_vel);
+/* Cell pdf <- feq(rho, u) (D2Q9.calc_eq, src/dynamics.py:70-74), the
+ * equilibrium initialisation Cells.__init__ uses for rho = 1, u = 0
+ * (src/containers.py:24-30) with given fields. */
+int fvdbm_set_equilibrium(void *handle, const void *cell_rho, const void *cell_vel);
+/* As fvdbm_set_equilibrium from node values: each cell takes the
+ * inverse-distance weighting (weights 1/d, d the node's distance to the
+ * cell centre as in the node lists) of its three vertices. */
+int fvdbm_set_equilibrium_nodes(void *handle, const void *node_rho, const void *node_vel);
 
 /* Device cell order: new_to_old[i] = reference cell id of device cell i. */
 int fvdbm_get_permutation(void *handle, int32_t *new_to_old);

@@ -82,3 +82,11 @@ def stable_dt(mesher, faces_L_scaled, tau, cfl=0.5):
     L = np.asarray(faces_L_scaled, np.float64).reshape(-1)
     per_cell = L[mesher.cell_face_indices].sum(axis=1) * np.sqrt(2.0)
     return cfl / (1.0 / tau + per_cell.max())
+
+
+def porous_case(seed, target_cells=8000, scheme="upwind", dist=None):
+    """One porous-channel case of the study distribution (3-8 discs, velocity
+    inlet, density outlet), returned as (mesher, dynamics, cells, faces, nodes)."""
+    c = fv.porous.make_case(seed, dist=dist or fv.porous.STUDY, target_cells=target_cells,
+                            scheme=scheme)
+    return c.mesher, c.dynamics, c.cells, c.faces, c.nodes

@@ -185,3 +185,20 @@ def test_every_cell_written_and_deterministic(gpu):
         env.close()
     assert np.isfinite(outs[0]).all()
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_checkpoint_resume_bitwise(gpu, tmp_path):
+    """N steps straight equal k steps, save, load, N - k steps (bitwise)."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=3)
+    env = _gpu_env(cells, faces, nodes)
+    env.step(60)
+    straight = env.cells.pdf
+    env.close()
+    env = _gpu_env(cells, faces, nodes)
+    env.step(25)
+    path = m.to_pickle(env, str(tmp_path / "ck"))
+    env.close()
+    env2, m2 = fv.Mesher.from_pickle(str(tmp_path / "ck"))
+    assert np.array_equal(m2.cells, m.cells)
+    env2.step(35)
+    assert np.array_equal(env2.cells.pdf, straight)
