@@ -86,6 +86,22 @@ _SIGNATURES = {
     "fvdbm_cell_fields": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "fvdbm_set_equilibrium": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "fvdbm_set_equilibrium_nodes": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_last_error": (ct.c_char_p, []),
+    "fvdbm_nn_chunk_rows": (ct.c_int, []),
+    "fvdbm_nn_mlp_forward": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_void_p,
+                                        ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_mlp_backward": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_wgrad": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_int32,
+                                  ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_colsum": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_void_p,
+                                   ct.c_void_p]),
+    "fvdbm_nn_aggregate": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                      ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_aggregate_backward": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+                                               ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                               ct.c_void_p]),
     "fvdbm_layout_order": (ct.c_int, [ct.POINTER(MeshDesc), ct.c_int32, ct.c_int32,
                                       ct.c_void_p, ct.c_void_p]),
 }

@@ -21,7 +21,7 @@ LIB = os.path.join(BUILD, "libfvdbm.so")
 
 SOURCES = [
     os.path.join(HERE, "csrc", "fvdbm_step.hip"),
-    os.path.join(HERE, "csrc", "fvdbm_batch.hip"),
+    os.path.join(HERE, "csrc", "surrogate.hip"),
     os.path.join(HERE, "csrc", "layout.cpp"),
 ]
 

@@ -1,0 +1,496 @@
+// Kernels of the warm-start surrogate: a mesh message-passing network whose
+// dense layers run on BF16 MFMA with FP32 accumulation, and the mesh
+// aggregation that couples the nodes. No reference counterpart (the
+// reference has no learned component); the C ABI is declared in
+// include/fvdbm.h (fvdbm_nn_*).
+//
+// Building blocks (all row-major float32 in HBM; weights are float32 master
+// copies, rounded to bf16 when staged in LDS):
+//   mlp2   y = silu(x W1 + b1) W2 + b2, x [M,K], W1 [K,64], W2 [64,NO]
+//   mlp2 backward (data part): recomputes z = x W1 + b1, then
+//          dz = (dy W2^T) * silu'(z), dx = dz W1^T; writes dz and h = silu(z)
+//   wgrad  out = A^T B over the rows (A [M,KA], B [M,NB]) as per-chunk
+//          partial products plus a fixed-order reduction (deterministic)
+//   colsum out = sum over rows, same chunking
+//   aggregate: z = [h | sum_e w h_j | sum_e w ex h_j | sum_e w ey h_j]
+//          over each node's CSR edge list, and its transpose for dh.
+//
+// MFMA use: __builtin_amdgcn_mfma_f32_16x16x16bf16_1k. For one 64-lane wave
+// with i = lane % 16 and g = lane / 16:
+//   A (16x16, MxK): lane holds A[i][4g .. 4g+3]
+//   B (16x16, KxN): lane holds B[4g .. 4g+3][i]
+//   D (16x16, MxN): lane holds D[4g + r][i], r = 0..3
+// The dense layers are computed transposed (hidden units as MFMA rows, data
+// rows as MFMA columns): then the D fragment of one layer is, lane by lane,
+// exactly the B fragment the next layer needs, so the hidden activations
+// never leave registers.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "fvdbm.h"
+
+#pragma clang fp contract(fast)
+
+namespace fvdbm_nn {
+
+constexpr int HID = 64;
+constexpr int CHUNK = 2048;  // rows per partial product in wgrad / colsum
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ s4 lds4(const unsigned short *p) {
+  return s4{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
+}
+
+__device__ __forceinline__ s4 bf4(float4 v) {
+  return s4{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
+}
+
+// ---------------------------------------------------------------------------
+// mlp2 forward: one wave per 16-row tile, weights for the whole block in LDS
+// ---------------------------------------------------------------------------
+template <int K, int NO>
+__global__ __launch_bounds__(256) void k_mlp_fwd(const float *__restrict__ x, int M,
+                                                 const float *__restrict__ w1,
+                                                 const float *__restrict__ b1,
+                                                 const float *__restrict__ w2,
+                                                 const float *__restrict__ b2,
+                                                 float *__restrict__ y) {
+  constexpr int KP = K + 4, HP = HID + 4;
+  __shared__ unsigned short sW1T[HID * KP];  // [hidden][k]
+  __shared__ unsigned short sW2T[NO * HP];   // [out][hidden]
+  __shared__ float sb1[HID], sb2[NO];
+  for (int e = threadIdx.x; e < K * HID; e += blockDim.x)
+    sW1T[(e % HID) * KP + e / HID] = f2bf(w1[e]);
+  for (int e = threadIdx.x; e < HID * NO; e += blockDim.x)
+    sW2T[(e % NO) * HP + e / NO] = f2bf(w2[e]);
+  if (threadIdx.x < HID) sb1[threadIdx.x] = b1[threadIdx.x];
+  if (threadIdx.x < NO) sb2[threadIdx.x] = b2[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ntiles = (M + 15) / 16;
+  for (int t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
+    const int row = t * 16 + i;
+    const bool ok = row < M;
+    f4 z[4];
+#pragma unroll
+    for (int hb = 0; hb < 4; ++hb) z[hb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+      const float4 xv = ok ? *reinterpret_cast<const float4 *>(x + (size_t)row * K + ks * 16 + 4 * g)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      const s4 b = bf4(xv);
+#pragma unroll
+      for (int hb = 0; hb < 4; ++hb)
+        z[hb] = mfma(lds4(sW1T + (hb * 16 + i) * KP + ks * 16 + 4 * g), b, z[hb]);
+    }
+    s4 hfrag[4];
+#pragma unroll
+    for (int hb = 0; hb < 4; ++hb) {
+      float h4[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) h4[r] = silu(z[hb][r] + sb1[hb * 16 + 4 * g + r]);
+      hfrag[hb] = bf4(make_float4(h4[0], h4[1], h4[2], h4[3]));
+    }
+#pragma unroll
+    for (int ob = 0; ob < NO / 16; ++ob) {
+      f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        acc = mfma(lds4(sW2T + (ob * 16 + i) * HP + ks * 16 + 4 * g), hfrag[ks], acc);
+      if (ok) {
+        const int o = ob * 16 + 4 * g;
+        *reinterpret_cast<float4 *>(y + (size_t)row * NO + o) =
+            make_float4(acc[0] + sb2[o], acc[1] + sb2[o + 1], acc[2] + sb2[o + 2], acc[3] + sb2[o + 3]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// mlp2 backward, data part: dz [M,64], h [M,64] and (optionally) dx [M,K]
+// ---------------------------------------------------------------------------
+template <int K, int NO>
+__global__ __launch_bounds__(256) void k_mlp_bwd(const float *__restrict__ x, int M,
+                                                 const float *__restrict__ w1,
+                                                 const float *__restrict__ b1,
+                                                 const float *__restrict__ w2,
+                                                 const float *__restrict__ dy,
+                                                 float *__restrict__ dx, float *__restrict__ dz,
+                                                 float *__restrict__ hout) {
+  constexpr int HP = HID + 4, NP = NO + 4;
+  __shared__ unsigned short sW1[K * HP];    // [k][hidden]: z (strided reads) and dx
+  __shared__ unsigned short sW2[HID * NP];  // [hidden][out]: dh = dy W2^T
+  __shared__ float sb1[HID];
+  for (int e = threadIdx.x; e < K * HID; e += blockDim.x) sW1[(e / HID) * HP + e % HID] = f2bf(w1[e]);
+  for (int e = threadIdx.x; e < HID * NO; e += blockDim.x) sW2[(e / NO) * NP + e % NO] = f2bf(w2[e]);
+  if (threadIdx.x < HID) sb1[threadIdx.x] = b1[threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int ntiles = (M + 15) / 16;
+  for (int t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
+    const int row = t * 16 + i;
+    const bool ok = row < M;
+    f4 z[4], dh[4];
+#pragma unroll
+    for (int hb = 0; hb < 4; ++hb) {
+      z[hb] = f4{0.f, 0.f, 0.f, 0.f};
+      dh[hb] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int ks = 0; ks < K / 16; ++ks) {
+      const float4 xv = ok ? *reinterpret_cast<const float4 *>(x + (size_t)row * K + ks * 16 + 4 * g)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      const s4 b = bf4(xv);
+#pragma unroll
+      for (int hb = 0; hb < 4; ++hb) {
+        const unsigned short *p = sW1 + (ks * 16 + 4 * g) * HP + hb * 16 + i;
+        const s4 a = s4{(short)p[0], (short)p[HP], (short)p[2 * HP], (short)p[3 * HP]};
+        z[hb] = mfma(a, b, z[hb]);
+      }
+    }
+    // dh^T [hidden][rows] = W2 [hidden][out] . dy^T [out][rows]
+#pragma unroll
+    for (int os = 0; os < NO / 16; ++os) {
+      const float4 dv = ok ? *reinterpret_cast<const float4 *>(dy + (size_t)row * NO + os * 16 + 4 * g)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      const s4 b = bf4(dv);
+#pragma unroll
+      for (int hb = 0; hb < 4; ++hb)
+        dh[hb] = mfma(lds4(sW2 + (hb * 16 + i) * NP + os * 16 + 4 * g), b, dh[hb]);
+    }
+    s4 dzfrag[4];
+#pragma unroll
+    for (int hb = 0; hb < 4; ++hb) {
+      float d4[4], h4[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float zz = z[hb][r] + sb1[hb * 16 + 4 * g + r];
+        h4[r] = silu(zz);
+        d4[r] = dh[hb][r] * dsilu(zz);
+      }
+      if (ok) {
+        const size_t o = (size_t)row * HID + hb * 16 + 4 * g;
+        *reinterpret_cast<float4 *>(dz + o) = make_float4(d4[0], d4[1], d4[2], d4[3]);
+        *reinterpret_cast<float4 *>(hout + o) = make_float4(h4[0], h4[1], h4[2], h4[3]);
+      }
+      dzfrag[hb] = bf4(make_float4(d4[0], d4[1], d4[2], d4[3]));
+    }
+    if (dx) {
+      // dx^T [k][rows] = W1 [k][hidden] . dz^T [hidden][rows]
+#pragma unroll
+      for (int kb = 0; kb < K / 16; ++kb) {
+        f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+          acc = mfma(lds4(sW1 + (kb * 16 + i) * HP + ks * 16 + 4 * g), dzfrag[ks], acc);
+        if (ok)
+          *reinterpret_cast<float4 *>(dx + (size_t)row * K + kb * 16 + 4 * g) =
+              make_float4(acc[0], acc[1], acc[2], acc[3]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// wgrad: partial[c] = A[rows of chunk c]^T B[rows of chunk c]  ([KA][NB])
+// One block per chunk; rows staged 64 at a time in LDS as bf16; the output
+// 16x16 blocks are dealt round-robin to the four waves.
+// ---------------------------------------------------------------------------
+template <int KA, int NB>
+__global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
+                                               const float *__restrict__ B, int M,
+                                               float *__restrict__ partial) {
+  constexpr int R = 64, AP = KA + 4, BP = NB + 4;
+  constexpr int NBLK = (KA / 16) * (NB / 16);
+  constexpr int PER = (NBLK + 3) / 4;
+  __shared__ unsigned short sA[R * AP];
+  __shared__ unsigned short sB[R * BP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  const int r0 = blockIdx.x * CHUNK, r1 = min(M, r0 + CHUNK);
+  f4 acc[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) acc[p] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int base = r0; base < r1; base += R) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < R * KA; e += blockDim.x) {
+      const int rr = e / KA, c = e % KA;
+      sA[rr * AP + c] = base + rr < r1 ? f2bf(A[(size_t)(base + rr) * KA + c]) : (unsigned short)0;
+    }
+    for (int e = threadIdx.x; e < R * NB; e += blockDim.x) {
+      const int rr = e / NB, c = e % NB;
+      sB[rr * BP + c] = base + rr < r1 ? f2bf(B[(size_t)(base + rr) * NB + c]) : (unsigned short)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ms = 0; ms < R / 16; ++ms) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int blk = wave + 4 * p;
+        if (blk < NBLK) {
+          const int kb = blk / (NB / 16), nb = blk % (NB / 16);
+          const int m = ms * 16 + 4 * g;
+          const s4 a = s4{(short)sA[(m + 0) * AP + kb * 16 + i], (short)sA[(m + 1) * AP + kb * 16 + i],
+                          (short)sA[(m + 2) * AP + kb * 16 + i], (short)sA[(m + 3) * AP + kb * 16 + i]};
+          const s4 b = s4{(short)sB[(m + 0) * BP + nb * 16 + i], (short)sB[(m + 1) * BP + nb * 16 + i],
+                          (short)sB[(m + 2) * BP + nb * 16 + i], (short)sB[(m + 3) * BP + nb * 16 + i]};
+          acc[p] = mfma(a, b, acc[p]);
+        }
+      }
+    }
+  }
+  float *out = partial + (size_t)blockIdx.x * KA * NB;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int blk = wave + 4 * p;
+    if (blk < NBLK) {
+      const int kb = blk / (NB / 16), nb = blk % (NB / 16);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(size_t)(kb * 16 + 4 * g + r) * NB + nb * 16 + i] = acc[p][r];
+    }
+  }
+}
+
+// partial[c][n] = sum of column n over the rows of chunk c (float32, row order)
+__global__ void k_colsum(const float *__restrict__ A, int M, int N, float *__restrict__ partial) {
+  const int n = threadIdx.x;
+  if (n >= N) return;
+  const int r0 = blockIdx.x * CHUNK, r1 = min(M, r0 + CHUNK);
+  float s = 0.f;
+  for (int r = r0; r < r1; ++r) s += A[(size_t)r * N + n];
+  partial[(size_t)blockIdx.x * N + n] = s;
+}
+
+__global__ void k_zero(float *__restrict__ p, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = 0.f;
+}
+
+// out[e] = sum over chunks c (in order) of partial[c][e]
+__global__ void k_reduce_chunks(const float *__restrict__ partial, int nchunks, int n,
+                                float *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; ++c) s += partial[(size_t)c * n + e];
+  out[e] = s;
+}
+
+// ---------------------------------------------------------------------------
+// aggregation: one wave per node, lane = feature
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_aggregate(const float *__restrict__ h, int n,
+                                                   const int32_t *__restrict__ off,
+                                                   const int32_t *__restrict__ col,
+                                                   const float *__restrict__ w,
+                                                   const float *__restrict__ ex,
+                                                   const float *__restrict__ ey,
+                                                   float *__restrict__ z) {
+  const int node = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+  if (node >= n) return;
+  float a0 = 0.f, ax = 0.f, ay = 0.f;
+  for (int e = off[node]; e < off[node + 1]; ++e) {
+    const float v = h[(size_t)col[e] * HID + f] * w[e];
+    a0 += v;
+    ax += v * ex[e];
+    ay += v * ey[e];
+  }
+  float *zr = z + (size_t)node * 4 * HID;
+  zr[f] = h[(size_t)node * HID + f];
+  zr[HID + f] = a0;
+  zr[2 * HID + f] = ax;
+  zr[3 * HID + f] = ay;
+}
+
+// dh[i] = dz[i][0:64] + sum over i's edges e = (i, j) of the transpose
+// weights (wt = weight of edge j -> i, (ext, eyt) its direction) times
+// dz[j][64:256]
+__global__ __launch_bounds__(256) void k_aggregate_bwd(const float *__restrict__ dz, int n,
+                                                       const int32_t *__restrict__ off,
+                                                       const int32_t *__restrict__ col,
+                                                       const float *__restrict__ wt,
+                                                       const float *__restrict__ ext,
+                                                       const float *__restrict__ eyt,
+                                                       float *__restrict__ dh) {
+  const int node = blockIdx.x * 4 + (threadIdx.x >> 6), f = threadIdx.x & 63;
+  if (node >= n) return;
+  float s = dz[(size_t)node * 4 * HID + f];
+  for (int e = off[node]; e < off[node + 1]; ++e) {
+    const float *dj = dz + (size_t)col[e] * 4 * HID;
+    s += wt[e] * (dj[HID + f] + ext[e] * dj[2 * HID + f] + eyt[e] * dj[3 * HID + f]);
+  }
+  dh[(size_t)node * HID + f] = s;
+}
+
+static thread_local std::string g_err;
+
+static int nn_fail(const std::string &m) {
+  g_err = m;
+  return FVDBM_ERR_INVALID;
+}
+
+static int hip_status(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return FVDBM_OK;
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return FVDBM_ERR_HIP;
+}
+
+static int grid_for(int64_t m) {
+  const int64_t tiles = (m + 15) / 16;
+  const int64_t blocks = (tiles + 3) / 4;
+  return (int)(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+}
+
+template <int K, int NO>
+static void launch_fwd(const float *x, int m, const float *w1, const float *b1, const float *w2,
+                       const float *b2, float *y, hipStream_t s) {
+  hipLaunchKernelGGL((k_mlp_fwd<K, NO>), dim3(grid_for(m)), dim3(256), 0, s, x, m, w1, b1, w2, b2, y);
+}
+
+template <int K, int NO>
+static void launch_bwd(const float *x, int m, const float *w1, const float *b1, const float *w2,
+                       const float *dy, float *dx, float *dz, float *h, hipStream_t s) {
+  hipLaunchKernelGGL((k_mlp_bwd<K, NO>), dim3(grid_for(m)), dim3(256), 0, s, x, m, w1, b1, w2, dy,
+                     dx, dz, h);
+}
+
+template <int KA, int NB>
+static void launch_wgrad(const float *a, const float *b, int m, float *partial, hipStream_t s) {
+  const int nch = (m + CHUNK - 1) / CHUNK;
+  hipLaunchKernelGGL((k_wgrad<KA, NB>), dim3(nch), dim3(256), 0, s, a, b, m, partial);
+}
+
+}  // namespace fvdbm_nn
+
+using namespace fvdbm_nn;
+
+extern "C" {
+
+const char *fvdbm_nn_last_error(void) { return g_err.c_str(); }
+
+int fvdbm_nn_chunk_rows(void) { return CHUNK; }
+
+#define NN_DISPATCH_K_NO(K_, NO_, CALL) \
+  if (k == K_ && n_out == NO_) {        \
+    CALL(K_, NO_);                      \
+    return hip_status("mlp launch");    \
+  }
+
+int fvdbm_nn_mlp_forward(const float *x, int64_t m, int32_t k, const float *w1, const float *b1,
+                         const float *w2, const float *b2, int32_t n_out, float *y, void *stream) {
+  if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
+  if (m == 0) return FVDBM_OK;
+  if (!x || !w1 || !b1 || !w2 || !b2 || !y) return nn_fail("null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mi = (int)m;
+#define CALL_FWD(K_, NO_) launch_fwd<K_, NO_>(x, mi, w1, b1, w2, b2, y, s)
+  NN_DISPATCH_K_NO(16, 64, CALL_FWD)
+  NN_DISPATCH_K_NO(64, 64, CALL_FWD)
+  NN_DISPATCH_K_NO(256, 64, CALL_FWD)
+  NN_DISPATCH_K_NO(64, 16, CALL_FWD)
+#undef CALL_FWD
+  return nn_fail("unsupported (in, out) widths: (16|64|256, 64) or (64, 16)");
+}
+
+int fvdbm_nn_mlp_backward(const float *x, int64_t m, int32_t k, const float *w1, const float *b1,
+                          const float *w2, int32_t n_out, const float *dy, float *dx, float *dz,
+                          float *h, void *stream) {
+  if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
+  if (m == 0) return FVDBM_OK;
+  if (!x || !w1 || !b1 || !w2 || !dy || !dz || !h) return nn_fail("null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mi = (int)m;
+#define CALL_BWD(K_, NO_) launch_bwd<K_, NO_>(x, mi, w1, b1, w2, dy, dx, dz, h, s)
+  NN_DISPATCH_K_NO(16, 64, CALL_BWD)
+  NN_DISPATCH_K_NO(64, 64, CALL_BWD)
+  NN_DISPATCH_K_NO(256, 64, CALL_BWD)
+  NN_DISPATCH_K_NO(64, 16, CALL_BWD)
+#undef CALL_BWD
+  return nn_fail("unsupported (in, out) widths: (16|64|256, 64) or (64, 16)");
+}
+
+int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_t nb, float *partial,
+                   float *out, void *stream) {
+  if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
+  if (!a || !b || !partial || !out) return nn_fail("null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mi = (int)m;
+  const int nch = (mi + CHUNK - 1) / CHUNK;
+  if (mi == 0) {
+    hipLaunchKernelGGL(k_zero, dim3((ka * nb + 255) / 256), dim3(256), 0, s, out, ka * nb);
+    return hip_status("zero launch");
+  }
+  if (ka == 16 && nb == 64) launch_wgrad<16, 64>(a, b, mi, partial, s);
+  else if (ka == 64 && nb == 64) launch_wgrad<64, 64>(a, b, mi, partial, s);
+  else if (ka == 256 && nb == 64) launch_wgrad<256, 64>(a, b, mi, partial, s);
+  else if (ka == 64 && nb == 16) launch_wgrad<64, 16>(a, b, mi, partial, s);
+  else return nn_fail("unsupported wgrad widths");
+  const int n = ka * nb;
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, nch, n, out);
+  return hip_status("wgrad launch");
+}
+
+int fvdbm_nn_colsum(const float *a, int64_t m, int32_t n, float *partial, float *out, void *stream) {
+  if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
+  if (!a || !partial || !out) return nn_fail("null pointer");
+  if (n < 1 || n > 256) return nn_fail("column count must be in [1, 256]");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mi = (int)m;
+  const int nch = (mi + CHUNK - 1) / CHUNK;
+  if (mi == 0) {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, out, n);
+    return hip_status("zero launch");
+  }
+  hipLaunchKernelGGL(k_colsum, dim3(nch), dim3(256), 0, s, a, mi, n, partial);
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, nch, n, out);
+  return hip_status("colsum launch");
+}
+
+int fvdbm_nn_aggregate(const float *h, int64_t n_nodes, const int32_t *row_off, const int32_t *col,
+                       const float *w, const float *ex, const float *ey, float *z, void *stream) {
+  if (n_nodes < 0 || n_nodes > INT32_MAX) return nn_fail("node count out of range");
+  if (n_nodes == 0) return FVDBM_OK;
+  if (!h || !row_off || !col || !w || !ex || !ey || !z) return nn_fail("null pointer");
+  const int n = (int)n_nodes;
+  hipLaunchKernelGGL(k_aggregate, dim3((n + 3) / 4), dim3(256), 0, static_cast<hipStream_t>(stream), h,
+                     n, row_off, col, w, ex, ey, z);
+  return hip_status("aggregate launch");
+}
+
+int fvdbm_nn_aggregate_backward(const float *dz, int64_t n_nodes, const int32_t *row_off,
+                                const int32_t *col, const float *wt, const float *ext,
+                                const float *eyt, float *dh, void *stream) {
+  if (n_nodes < 0 || n_nodes > INT32_MAX) return nn_fail("node count out of range");
+  if (n_nodes == 0) return FVDBM_OK;
+  if (!dz || !row_off || !col || !wt || !ext || !eyt || !dh) return nn_fail("null pointer");
+  const int n = (int)n_nodes;
+  hipLaunchKernelGGL(k_aggregate_bwd, dim3((n + 3) / 4), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), dz, n, row_off, col, wt, ext, eyt, dh);
+  return hip_status("aggregate backward launch");
+}
+
+}  // extern "C"

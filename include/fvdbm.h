@@ -214,6 +214,45 @@ int fvdbm_get_permutation(void *handle, int32_t *new_to_old);
 int fvdbm_layout_order(const fvdbm_mesh_desc *mesh, int32_t precision, int32_t reorder,
                        int32_t *new_to_old, int32_t *stats);
 
+/* ------------------------------------------------------------------------
+ * Warm-start surrogate kernels (no reference counterpart: the reference has
+ * no learned component). Device pointers to row-major float32 arrays; the
+ * stream is a hipStream_t (NULL = the default stream). Dense layers use
+ * bf16 MFMA operands with float32 accumulation; weights are float32 master
+ * copies rounded to bf16 on load. Hidden width 64.
+ * ------------------------------------------------------------------------ */
+const char *fvdbm_nn_last_error(void);
+/* Rows per partial product of fvdbm_nn_wgrad / fvdbm_nn_colsum: their
+ * `partial` workspace holds ceil(m / rows) * (output size) floats. */
+int fvdbm_nn_chunk_rows(void);
+/* y [m][n_out] = silu(x [m][k] w1 [k][64] + b1) w2 [64][n_out] + b2;
+ * (k, n_out) in {(16,64), (64,64), (256,64), (64,16)}. */
+int fvdbm_nn_mlp_forward(const float *x, int64_t m, int32_t k, const float *w1, const float *b1,
+                         const float *w2, const float *b2, int32_t n_out, float *y, void *stream);
+/* Data part of the backward pass of fvdbm_nn_mlp_forward: with
+ * z = x w1 + b1 recomputed, dz [m][64] = (dy w2^T) * silu'(z),
+ * h [m][64] = silu(z) and, unless dx is NULL, dx [m][k] = dz w1^T. */
+int fvdbm_nn_mlp_backward(const float *x, int64_t m, int32_t k, const float *w1, const float *b1,
+                          const float *w2, int32_t n_out, const float *dy, float *dx, float *dz,
+                          float *h, void *stream);
+/* out [ka][nb] = a [m][ka]^T b [m][nb]; (ka, nb) as for the layers.
+ * Deterministic: per-chunk partial products, summed in chunk order. */
+int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_t nb, float *partial,
+                   float *out, void *stream);
+/* out [n] = column sums of a [m][n] (n <= 256), deterministic as above. */
+int fvdbm_nn_colsum(const float *a, int64_t m, int32_t n, float *partial, float *out, void *stream);
+/* Mesh aggregation over a CSR edge list (row_off [n+1], col [E]):
+ * z [n][256] = [h_i | sum_e w_e h_j | sum_e w_e ex_e h_j | sum_e w_e ey_e h_j],
+ * h [n][64]. */
+int fvdbm_nn_aggregate(const float *h, int64_t n_nodes, const int32_t *row_off, const int32_t *col,
+                       const float *w, const float *ex, const float *ey, float *z, void *stream);
+/* Its transpose: dh_i = dz_i[0:64] + sum over i's edges e = (i, j) of
+ * wt_e (dz_j[64:128] + ext_e dz_j[128:192] + eyt_e dz_j[192:256]), where
+ * (wt, ext, eyt) are the weight and direction of the reverse edge j -> i. */
+int fvdbm_nn_aggregate_backward(const float *dz, int64_t n_nodes, const int32_t *row_off,
+                                const int32_t *col, const float *wt, const float *ext,
+                                const float *eyt, float *dh, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

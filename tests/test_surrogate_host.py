@@ -1,0 +1,66 @@
+"""Host parts of the surrogate, on the CPU: the node graph built for the
+aggregation kernels, the CPU twin's aggregation against a dense adjacency
+construction, and the twin's bfloat16 rounding against the kernels'
+round-to-nearest-even bit formula."""
+import importlib
+
+import numpy as np
+import torch
+
+from cases import fv
+from oracle import surrogate_twin as tw
+
+sg = importlib.import_module("fvdbm-jax_amd.surrogate")
+
+
+def _case():
+    c = fv.porous.make_case(2, target_cells=1200)
+    return c, sg.case_inputs(c)
+
+
+def test_graph_arrays():
+    c, (x, g) = _case()
+    N = c.n_nodes
+    assert x.shape == (N, 16) and np.all(x[:, 9:] == 0)
+    off, col = g["off"], g["col"]
+    src = np.repeat(np.arange(N), np.diff(off))
+    pairs = set(zip(src.tolist(), col.tolist()))
+    assert all((j, i) in pairs for i, j in pairs)            # symmetric
+    assert len(pairs) == 2 * len(c.mesher.faces)             # every face, both ways
+    assert np.all(np.diff(col.astype(np.int64) + src.astype(np.int64) * N) > 0)  # sorted, unique
+    wsum = np.add.reduceat(g["w"].astype(np.float64), off[:-1])
+    assert np.allclose(wsum, 1.0, atol=1e-6)                 # mean weights
+    assert np.allclose(g["ex"] ** 2 + g["ey"] ** 2, 1.0, atol=1e-6)
+    # the transpose arrays describe the reverse edge
+    deg = np.diff(off)
+    assert np.allclose(g["wt"], 1.0 / deg[col]) and np.array_equal(g["ext"], -g["ex"])
+
+
+def test_twin_aggregate_matches_dense_adjacency():
+    c, (x, g) = _case()
+    N = c.n_nodes
+    p = c.mesher.points
+    A = np.zeros((N, N))
+    DX = np.zeros((N, N))
+    DY = np.zeros((N, N))
+    for a, b in np.asarray(c.mesher.faces):
+        for i, j in ((a, b), (b, a)):
+            d = p[j] - p[i]
+            L = np.hypot(*d)
+            A[i, j], DX[i, j], DY[i, j] = 1.0, d[0] / L, d[1] / L
+    deg = A.sum(1, keepdims=True)
+    h = np.random.default_rng(0).standard_normal((N, 64))
+    want = np.concatenate([h, (A / deg) @ h, (A * DX / deg) @ h, (A * DY / deg) @ h], 1)
+    got = tw.aggregate(torch.tensor(h), g).numpy()
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+def test_bf16_rounding_matches_kernel_formula():
+    rng = np.random.default_rng(0)
+    f = np.concatenate([rng.standard_normal(100000).astype(np.float32),
+                        np.array([1.00390625, 1.01171875, -2.0039062, 3.0e-39], np.float32)])
+    u = f.view(np.uint32).astype(np.uint64)
+    bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) << 16
+    want = bits.astype(np.uint32).view(np.float32)
+    got = tw._round_bf16(torch.tensor(f)).numpy()
+    assert np.array_equal(got, want)
