@@ -14,7 +14,16 @@ elapsed / 1e6 (MLUPS). roofline.achieved = 108 B/LUP * cells / (elapsed/K).
 cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
 of the same mesh, on this host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
+steady_state (the second half of the metric): ``--steady-cases`` unseen
+porous-channel cases of the study distribution (seeds from 1,000,000; the
+surrogate was trained on seeds below 10,000) solved as one batch to a
+residual of 1e-6, first from the uniform state (cold), then from the
+surrogate's prediction (warm, cold fallback per case). Reported: steps per
+case, batch wall time (the warm time includes meshing-independent feature
+preparation and inference), and the relative L2 distance of the final
+cell fields.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--steady-cases M]
 """
 from __future__ import annotations
 
@@ -97,6 +106,53 @@ def cpu_baseline(m, dyn, c, f, n, budget_s=12.0):
                        f"(oracle/fvdbm_oracle.c, gcc -O2), 1 thread, {el:.1f}s")
 
 
+WEIGHTS = os.path.join(ROOT, "fvdbm-jax_amd", "data", "surrogate_porous.npz")
+
+
+def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
+    """Cold against warm-started batched solves of the same cases."""
+    fv = importlib.import_module("fvdbm-jax_amd")
+    batch = importlib.import_module("fvdbm-jax_amd.batch")
+    ws = importlib.import_module("fvdbm-jax_amd.warmstart")
+    sg = importlib.import_module("fvdbm-jax_amd.surrogate")
+    b = batch.BatchSolver([c.env_tuple() for c in cases])
+    # a short run first so graph capture and first-touch are outside both timings
+    ws.solve(b, None, tol=tol, check_every=check_every, max_steps=2 * check_every)
+    cold = ws.solve(b, None, tol=tol, check_every=check_every, max_steps=max_steps)
+    cr, cv = b.cell_fields()
+    # the best a node-field predictor can do: restart from the converged
+    # solution's own node fields
+    ideal = ws.solve(b, b.node_fields(), tol=tol, check_every=check_every, max_steps=max_steps)
+    net = sg.MeshNet.load(weights)
+    t0 = time.perf_counter()
+    pred = net.predict(cases)
+    t_pred = time.perf_counter() - t0
+    init = (np.concatenate([p[0] for p in pred]), np.concatenate([p[1] for p in pred]))
+    warm = ws.solve(b, init, tol=tol, check_every=check_every, max_steps=max_steps)
+    wr, wv = b.cell_fields()
+    du = np.array([ws.relative_l2(b.case_cells(k, wv), b.case_cells(k, cv)) for k in range(b.n)])
+    dr = np.array([ws.relative_l2(b.case_cells(k, wr) - 1.0, b.case_cells(k, cr) - 1.0)
+                   for k in range(b.n)])
+    b.close()
+    ok = (cold["status"] == ws.CONVERGED) & (warm["status"] == ws.CONVERGED)
+    ratio = cold["steps"][ok] / np.maximum(warm["steps"][ok], 1)
+    warm_s = warm["wall_s"] + t_pred
+    return dict(
+        cases=len(cases), cells=int(sum(c.n_cells for c in cases)), tol=tol, check_every=check_every,
+        converged_cold=int((cold["status"] == ws.CONVERGED).sum()),
+        converged_warm=int((warm["status"] == ws.CONVERGED).sum()),
+        fallbacks=int(warm["fallback"].sum()),
+        steps_cold_median=float(np.median(cold["steps"])), steps_warm_median=float(np.median(warm["steps"])),
+        steps_cold_total=int(cold["steps"].sum()), steps_warm_total=int(warm["steps"].sum()),
+        steps_exact_fields_median=float(np.median(ideal["steps"])),
+        step_ratio_median=float(np.median(ratio)) if ratio.size else None,
+        step_ratio_total=float(cold["steps"].sum() / max(warm["steps"].sum(), 1)),
+        wall_cold_s=round(cold["wall_s"], 3), wall_warm_s=round(warm_s, 3),
+        inference_s=round(t_pred, 3), wall_ratio=round(cold["wall_s"] / warm_s, 3),
+        final_rel_l2_u_median=float(np.median(du)), final_rel_l2_u_max=float(du.max()),
+        final_rel_l2_rho_median=float(np.median(dr)), final_rel_l2_rho_max=float(dr.max()))
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
@@ -121,11 +177,23 @@ def main():
                     help="also time a 256 MiB device copy (counter calibration)")
     ap.add_argument("--mesh-cache", default=None,
                     help="directory to keep the generated mesh between runs")
+    ap.add_argument("--steady-cases", type=int, default=64,
+                    help="unseen porous cases for the cold/warm steady-state study (0: skip)")
+    ap.add_argument("--steady-seed0", type=int, default=1_000_000)
+    ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    steady_cases = None
+    if args.steady_cases > 0 and os.path.exists(args.weights):
+        # meshed in a fork pool before anything touches the GPU
+        ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
+        t0 = time.time()
+        steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases))
+        log(f"steady-state study: {len(steady_cases)} cases meshed in {time.time() - t0:.1f}s")
 
     copy = None
     if args.copy_probe:
@@ -163,8 +231,14 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=(tr["bytes_per_launch"] if tr else None))
     cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
+    env.close()
+    steady = None
+    if steady_cases is not None:
+        t0 = time.time()
+        steady = steady_state(steady_cases, args.weights)
+        log(f"steady-state study {time.time() - t0:.1f}s: {steady}")
     out = {
-        "metric": "MLUPS at 1M cells (%HBM roofline)",
+        "metric": "MLUPS at 1M cells (%HBM roofline); time-to-steady-state, warm vs cold start",
         "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms / args.steps, 5), "higher_is_better": True,
@@ -177,6 +251,7 @@ def main():
                    "residual_after": r},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "steady_state": steady,
         **({"copy_probe": copy} if copy else {}),
     }
     print(json.dumps(out), flush=True)

@@ -137,6 +137,25 @@ def node_features(case):
     return out
 
 
+def section_features(points, discs, grid=2048):
+    """Per-node cross-section descriptors of the channel at the node's x:
+    open fraction a(x) = 1 - (sum of disc chords)/HEIGHT, 1/a(x), the share
+    of the total hydraulic resistance int 1/a^3 dx lying downstream of x,
+    and log of that total. [N, 4] float32."""
+    xs = (np.arange(grid) + 0.5) / grid
+    blocked = np.zeros(grid)
+    for cx, cy, r in discs:
+        d2 = r * r - (xs - cx) ** 2
+        blocked += np.where(d2 > 0, 2.0 * np.sqrt(np.maximum(d2, 0.0)), 0.0)
+    a = np.clip(1.0 - blocked / HEIGHT, 0.05, 1.0)
+    res = 1.0 / a ** 3
+    total = res.sum() / grid
+    down = np.cumsum(res[::-1])[::-1] / grid / total
+    px = np.asarray(points, np.float64)[:, 0]
+    i = np.clip((px * grid).astype(np.int64), 0, grid - 1)
+    return np.stack([a[i], 1.0 / a[i], down[i], np.full(len(px), np.log(total))], 1).astype(np.float32)
+
+
 def mesh_edges(case):
     """Undirected node pairs of the mesh (its faces), [F, 2] int32."""
     return np.asarray(case.mesher.faces, np.int32)

@@ -4,13 +4,19 @@ steady (rho - 1, u) of a porous-channel case at every mesh node.
 The network (``MeshNet``) works on the node graph of the mesh (its faces as
 undirected edges):
 
-  h0   = mlp_enc(x)                      x: 9 node features padded to 16
+  h0   = mlp_enc(x)                      x: 9 node features (+2 flags) padded to 16
   z_l  = [h | mean_j h_j | mean_j ex_ij h_j | mean_j ey_ij h_j]     (256)
   h_l+1 = h_l + mlp_l(z_l)               l = 0..5
   y    = mlp_dec(h_6)[:, :3]             (rho - 1, ux, uy), normalised
 
 with mlp(x) = silu(x W1 + b1) W2 + b2, hidden width 64, and (ex, ey) the
-unit vector along edge i -> j. Every dense layer, its backward pass, the
+unit vector along edge i -> j. Besides the mesh nodes the graph holds
+virtual nodes that give every layer a long reach: SLABS "slab" nodes, each
+linked both ways to the mesh nodes in its band of x and to its neighbouring
+slabs, and one "global" node linked both ways to every slab. Virtual nodes
+carry flags in input columns 9 (slab) and 10 (global) and no targets.
+Targets are (rho - 1, ux, uy) divided by the case's inlet speed and by a
+per-component scale fixed from the training set. Every dense layer, its backward pass, the
 weight gradients and the aggregation are this library's HIP kernels
 (fvdbm_nn_*: bf16 MFMA operands, float32 accumulation, float32 master
 weights) behind ``torch.autograd.Function``; PyTorch only holds the tensors,
@@ -33,8 +39,8 @@ OUT_PAD = 16
 LAYERS = 6
 
 # fixed input scaling: (x - shift) / scale per feature column
-FEATURE_SHIFT = np.array([0.5, 0.25, 0.05, 0, 0, 0, 0, 0.035, 0.725], np.float32)
-FEATURE_SCALE = np.array([0.29, 0.144, 0.05, 1, 1, 1, 1, 0.01, 0.1], np.float32)
+FEATURE_SHIFT = np.array([0.5, 0.25, 0.126, 0, 0, 0, 0, 0.035, 0.725], np.float32)
+FEATURE_SCALE = np.array([0.29, 0.144, 0.095, 1, 1, 1, 1, 0.009, 0.1], np.float32)
 
 
 def _torch():
@@ -63,26 +69,17 @@ def _stream():
 # graphs
 # ---------------------------------------------------------------------------
 def graph_arrays(points, edges):
-    """CSR node graph of a mesh: both directions of every edge, sorted by
-    (source, target). Returns dict of NumPy arrays: off [N+1], col [E],
-    w = 1/deg(source), (ex, ey) = unit(p_target - p_source), and for the
-    backward pass the reverse edge's weight wt = 1/deg(target) and
-    direction (ext, eyt) = -(ex, ey)."""
+    """CSR node graph of a mesh alone: both directions of every edge,
+    sorted by (source, target), w = 1/deg(source), (ex, ey) =
+    unit(p_target - p_source), and for the backward pass the reverse edge's
+    weight wt = 1/deg(target) and direction (ext, eyt) = -(ex, ey)."""
     pts = np.asarray(points, np.float64)
     e = np.asarray(edges, np.int64)
-    e = np.concatenate([e, e[:, ::-1]])
-    e = e[np.lexsort((e[:, 1], e[:, 0]))]
-    src, dst = e[:, 0], e[:, 1]
-    N = len(pts)
-    deg = np.bincount(src, minlength=N)
-    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
     d = pts[dst] - pts[src]
     L = np.hypot(d[:, 0], d[:, 1])
-    ex, ey = d[:, 0] / L, d[:, 1] / L
-    return dict(off=off, col=dst.astype(np.int32), w=(1.0 / deg[src]).astype(np.float32),
-                ex=ex.astype(np.float32), ey=ey.astype(np.float32),
-                wt=(1.0 / deg[dst]).astype(np.float32), ext=(-ex).astype(np.float32),
-                eyt=(-ey).astype(np.float32), n=N)
+    return graph_from_edges(len(pts), src, dst, d[:, 0] / L, d[:, 1] / L)
 
 
 def concat_graphs(graphs):
@@ -108,12 +105,95 @@ def graph_to_device(g, device="cuda"):
             for k, v in g.items()}
 
 
-def case_inputs(case):
-    """Scaled and padded node inputs [N, 16] of a porous.Case, and its graph."""
+SLABS = 32
+
+
+def graph_from_edges(n, src, dst, ex, ey):
+    """CSR graph from directed edges (any order): rows sorted by (source,
+    target); weights 1/deg(source); transpose arrays for the backward pass
+    from the reverse edge (which must exist)."""
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    ex = np.asarray(ex, np.float64)[order]
+    ey = np.asarray(ey, np.float64)[order]
+    deg = np.bincount(src, minlength=n)
+    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    # the reverse edge of (s, d) is (d, s): find it by key
+    key = src * n + dst
+    rkey = dst * n + src
+    rev = np.searchsorted(key, rkey)
+    if not np.array_equal(key[rev], rkey):
+        raise ValueError("graph edges must come in both directions")
+    return dict(off=off, col=dst.astype(np.int32), w=(1.0 / deg[src]).astype(np.float32),
+                ex=ex.astype(np.float32), ey=ey.astype(np.float32),
+                wt=(1.0 / deg[dst]).astype(np.float32), ext=ex[rev].astype(np.float32),
+                eyt=ey[rev].astype(np.float32), n=int(n))
+
+
+SECTION_SHIFT = np.array([0.95, 1.06, 0.5, 0.26], np.float32)
+SECTION_SCALE = np.array([0.085, 0.116, 0.31, 0.106], np.float32)
+
+
+def case_inputs(case, slabs=SLABS, section=True):
+    """Scaled and padded node inputs [N + slabs + 1, 16] of a porous.Case
+    (mesh nodes first, then slab nodes, then the global node), its graph,
+    and the number of mesh nodes. Columns 0-8: the node features, 9-10:
+    virtual-node flags, 11-14 (with ``section``): porous.section_features."""
     f = porous.node_features(case)
-    x = np.zeros((f.shape[0], IN_PAD), np.float32)
-    x[:, :f.shape[1]] = (f - FEATURE_SHIFT) / FEATURE_SCALE
-    return x, graph_arrays(case.mesher.points, porous.mesh_edges(case))
+    pts = np.asarray(case.mesher.points, np.float64)
+    N = f.shape[0]
+    S = slabs
+    n = N + S + 1
+    x = np.zeros((n, IN_PAD), np.float32)
+    x[:N, :f.shape[1]] = (f - FEATURE_SHIFT) / FEATURE_SCALE
+    if section:
+        vx = np.concatenate([pts[:, 0], (np.arange(S) + 0.5) / S, [0.5]])
+        sf = porous.section_features(np.stack([vx, np.zeros_like(vx)], 1), case.discs)
+        x[:, 11:15] = (sf - SECTION_SHIFT) / SECTION_SCALE
+    xc = (np.arange(S) + 0.5) / S
+    virt = np.zeros((S + 1, f.shape[1]), np.float32)
+    virt[:S, 0] = xc
+    virt[S, 0] = 0.5
+    virt[:, 1] = 0.25
+    virt[:, 2] = FEATURE_SHIFT[2]
+    virt[:, 7] = case.u_in
+    virt[:, 8] = case.dynamics.tau
+    x[N:, :f.shape[1]] = (virt - FEATURE_SHIFT) / FEATURE_SCALE
+    x[N:N + S, 9] = 1.0
+    x[N + S, 10] = 1.0
+    # mesh edges
+    e = porous.mesh_edges(case).astype(np.int64)
+    d = pts[e[:, 1]] - pts[e[:, 0]]
+    L = np.hypot(d[:, 0], d[:, 1])
+    src = [e[:, 0], e[:, 1]]
+    dst = [e[:, 1], e[:, 0]]
+    exs = [d[:, 0] / L, -d[:, 0] / L]
+    eys = [d[:, 1] / L, -d[:, 1] / L]
+    # node <-> its slab (no direction)
+    slab = np.minimum((pts[:, 0] * S).astype(np.int64), S - 1) + N
+    nodes = np.arange(N)
+    z = np.zeros(N)
+    src += [nodes, slab]
+    dst += [slab, nodes]
+    exs += [z, z]
+    eys += [z, z]
+    # slab <-> next slab (along x), slab <-> global
+    a = np.arange(S - 1) + N
+    src += [a, a + 1]
+    dst += [a + 1, a]
+    exs += [np.ones(S - 1), -np.ones(S - 1)]
+    eys += [np.zeros(S - 1), np.zeros(S - 1)]
+    sl = np.arange(S) + N
+    gnode = np.full(S, N + S)
+    src += [sl, gnode]
+    dst += [gnode, sl]
+    exs += [np.zeros(S), np.zeros(S)]
+    eys += [np.zeros(S), np.zeros(S)]
+    g = graph_from_edges(n, np.concatenate(src), np.concatenate(dst), np.concatenate(exs),
+                         np.concatenate(eys))
+    return x, g, N
 
 
 # ---------------------------------------------------------------------------
@@ -291,55 +371,136 @@ class MeshNet:
     @classmethod
     def load(cls, path):
         z = np.load(path)
-        params = {k: z[k] for k in z.files}
+        params = {k: z[k] for k in z.files if k != "record"}
         return cls(params, layers=int(z["layers"]))
 
-    def predict(self, cases):
-        """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2])."""
+    def predict(self, cases, project=False):
+        """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
+        with ``project``, u_x passes through ``project_mass_flux``."""
         torch = _torch()
-        xs, gs = zip(*[case_inputs(c) for c in cases])
-        g = graph_to_device(concat_graphs(gs))
-        x = torch.from_numpy(np.concatenate(xs)).cuda()
+        ins = [case_inputs(c) for c in cases]
+        g = graph_to_device(concat_graphs([i[1] for i in ins]))
+        x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
         with torch.no_grad():
             y = self.forward(x, g).cpu().numpy().astype(np.float64) * self.target_scale
         out, o = [], 0
-        for c in cases:
-            n = c.n_nodes
-            yy = y[o:o + n]
-            o += n
-            out.append(((1.0 + yy[:, 0]).astype(np.float32), yy[:, 1:3].astype(np.float32)))
+        for c, (_, gi, N) in zip(cases, ins):
+            yy = y[o:o + N] * c.u_in
+            o += gi["n"]
+            r = (1.0 + yy[:, 0]).astype(np.float32)
+            v = yy[:, 1:3].astype(np.float32)
+            out.append((r, project_mass_flux(c, r, v) if project else v))
         return out
 
 
-def targets(rho, vel, scale):
-    """Scaled training targets [(rho - 1)/s0, ux/s1, uy/s2]."""
+def project_mass_flux(case, rho, vel, bins=64):
+    """Rescale a predicted u_x band by band along x so that each band
+    carries the inlet's mass flux: for the parabolic inlet the flux is
+    (2/3) u_in HEIGHT times the mean inlet density, and a band's flux is
+    estimated as the mean of rho u_x over its nodes times its open height
+    a(x) HEIGHT. The per-band factors (clipped to [0.5, 2]) are interpolated
+    linearly in x between band centres. Returns the corrected velocity."""
+    pts = np.asarray(case.mesher.points, np.float64)
+    rho = np.asarray(rho, np.float64)
+    vel = np.array(vel, np.float64)
+    inlet = case.mesher.point_markers == porous.meshgen.INLET
+    q = (2.0 / 3.0) * case.u_in * porous.HEIGHT * (rho[inlet].mean() if inlet.any() else 1.0)
+    k = np.minimum((pts[:, 0] * bins).astype(np.int64), bins - 1)
+    centres = (np.arange(bins) + 0.5) / bins
+    a = porous.section_features(np.stack([centres, np.zeros(bins)], 1), case.discs)[:, 0]
+    num = np.bincount(k, weights=rho * vel[:, 0], minlength=bins)
+    cnt = np.bincount(k, minlength=bins)
+    flux = np.where(cnt > 0, num / np.maximum(cnt, 1), 0.0) * a * porous.HEIGHT
+    fac = np.where(flux > 0, q / np.where(flux > 0, flux, 1.0), 1.0)
+    fac = np.clip(fac, 0.5, 2.0)
+    vel[:, 0] *= np.interp(pts[:, 0], centres, fac)
+    return vel.astype(np.float32)
+
+
+def targets(rho, vel, u_in, scale, n_virtual=0):
+    """Training targets [(rho - 1), ux, uy] / u_in / scale per component,
+    with zero rows for the virtual nodes."""
     t = np.stack([np.asarray(rho, np.float64) - 1.0, vel[:, 0], vel[:, 1]], axis=1)
-    return (t / np.asarray(scale, np.float64)).astype(np.float32)
+    t = t / u_in / np.asarray(scale, np.float64)
+    return np.concatenate([t, np.zeros((n_virtual, 3))]).astype(np.float32)
 
 
-def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None):
-    """Adam on the mean squared error over nodes. samples: list of
-    (x [N,16], graph_arrays, y [N,3]) with y already scaled. Returns the
-    per-epoch mean loss."""
+def device_sample(x, g, y, n_mesh=None):
+    """Upload one (x, graph, y) sample to the GPU once; rows from n_mesh on
+    (virtual nodes) are masked out of the loss."""
     torch = _torch()
+    d = graph_to_device(g)
+    d["x"] = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    d["y"] = torch.from_numpy(np.ascontiguousarray(y)).cuda()
+    m = np.zeros((g["n"], 1), np.float32)
+    m[:g["n"] if n_mesh is None else n_mesh] = 1.0
+    d["mask"] = torch.from_numpy(m).cuda()
+    return d
+
+
+def concat_device(samples):
+    """Block-diagonal batch of device samples: (x, graph, y)."""
+    torch = _torch()
+    offs, cols = [], []
+    n0 = e0 = 0
+    for s in samples:
+        offs.append(s["off"][:-1] + e0)
+        cols.append(s["col"] + n0)
+        n0 += s["n"]
+        e0 += int(s["col"].shape[0])
+    dev = samples[0]["x"].device
+    g = {"off": torch.cat(offs + [torch.tensor([e0], dtype=torch.int32, device=dev)]),
+         "col": torch.cat(cols), "n": n0}
+    for k in ("w", "ex", "ey", "wt", "ext", "eyt"):
+        g[k] = torch.cat([s[k] for s in samples])
+    return (torch.cat([s["x"] for s in samples]), g, torch.cat([s["y"] for s in samples]),
+            torch.cat([s["mask"] for s in samples]))
+
+
+def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final=1e-5):
+    """Adam on the mean squared error over mesh nodes, learning rate from
+    ``lr`` to ``lr_final`` on a cosine over the steps. samples: list of
+    (x, graph, y[, n_mesh]) with y already scaled, or of ``device_sample``
+    dicts. Returns the per-epoch mean loss."""
+    torch = _torch()
+    samples = [s if isinstance(s, dict) else device_sample(*s) for s in samples]
     opt = torch.optim.Adam(net.parameters(), lr=lr)
+    per_epoch = -(-len(samples) // batch)
+    total = max(1, epochs * per_epoch)
     rng = np.random.default_rng(seed)
     hist = []
+    it = 0
     for ep in range(epochs):
         order = rng.permutation(len(samples))
-        tot, cnt = 0.0, 0
+        losses = []
+        t_ep = __import__("time").perf_counter()
         for b0 in range(0, len(order), batch):
-            sel = [samples[i] for i in order[b0:b0 + batch]]
-            x = torch.from_numpy(np.concatenate([s[0] for s in sel])).cuda()
-            y = torch.from_numpy(np.concatenate([s[2] for s in sel])).cuda()
-            g = graph_to_device(concat_graphs([s[1] for s in sel]))
+            for grp in opt.param_groups:
+                grp["lr"] = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / total))
+            x, g, y, m = concat_device([samples[i] for i in order[b0:b0 + batch]])
             opt.zero_grad(set_to_none=True)
-            loss = torch.mean((net(x, g) - y) ** 2)
+            loss = torch.sum(m * (net(x, g) - y) ** 2) / (3.0 * torch.sum(m))
             loss.backward()
             opt.step()
-            tot += float(loss.detach()) * len(sel)
-            cnt += len(sel)
-        hist.append(tot / cnt)
+            losses.append(loss.detach())
+            it += 1
+        hist.append(float(torch.stack(losses).mean()))
         if log:
-            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e}")
+            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e} "
+                f"({__import__('time').perf_counter() - t_ep:.1f} s)")
     return hist
+
+
+def evaluate(net, samples):
+    """Relative L2 error of the predicted (rho - 1, ux, uy) on the mesh nodes
+    of each sample, as an array [len(samples), 3]."""
+    torch = _torch()
+    samples = [s if isinstance(s, dict) else device_sample(*s) for s in samples]
+    out = []
+    with torch.no_grad():
+        for s in samples:
+            m = s["mask"][:, 0] > 0
+            p = net(s["x"], s)[m]
+            t = s["y"][m]
+            out.append((torch.linalg.norm(p - t, dim=0) / torch.linalg.norm(t, dim=0)).cpu().numpy())
+    return np.array(out)

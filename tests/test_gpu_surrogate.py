@@ -75,7 +75,7 @@ def test_mlp_forward_backward(gpu, k, no, m):
 
 def _small_graph(seed=0):
     c = fv.porous.make_case(seed, target_cells=1500)
-    x, g = sg.case_inputs(c)
+    x, g, nm = sg.case_inputs(c)
     return c, x, g
 
 
@@ -148,7 +148,8 @@ def test_training_reduces_loss(gpu):
         c, x, g = _small_graph(s)
         p = c.mesher.points
         y = np.stack([np.sin(3 * p[:, 0]), p[:, 1] * 2, np.cos(2 * p[:, 0] + p[:, 1])], 1)
-        samples.append((x, g, y.astype(np.float32)))
+        y = np.concatenate([y, np.zeros((g["n"] - len(p), 3))])
+        samples.append((x, g, y.astype(np.float32), len(p)))
     net = sg.MeshNet(sg.init_params(seed=0, layers=2), layers=2)
     hist = sg.train(net, samples, epochs=30, batch=2, lr=3e-3)
     assert hist[-1] < 0.2 * hist[0], hist

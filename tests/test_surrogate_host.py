@@ -15,13 +15,12 @@ sg = importlib.import_module("fvdbm-jax_amd.surrogate")
 
 def _case():
     c = fv.porous.make_case(2, target_cells=1200)
-    return c, sg.case_inputs(c)
+    return c, sg.graph_arrays(c.mesher.points, c.mesher.faces)
 
 
 def test_graph_arrays():
-    c, (x, g) = _case()
+    c, g = _case()
     N = c.n_nodes
-    assert x.shape == (N, 16) and np.all(x[:, 9:] == 0)
     off, col = g["off"], g["col"]
     src = np.repeat(np.arange(N), np.diff(off))
     pairs = set(zip(src.tolist(), col.tolist()))
@@ -36,8 +35,34 @@ def test_graph_arrays():
     assert np.allclose(g["wt"], 1.0 / deg[col]) and np.array_equal(g["ext"], -g["ex"])
 
 
+def test_virtual_nodes():
+    c, _ = _case()
+    x, g, N = sg.case_inputs(c)
+    S = sg.SLABS
+    assert N == c.n_nodes and g["n"] == N + S + 1 and x.shape == (N + S + 1, 16)
+    assert np.all(x[:N, 9:] == 0) and np.all(x[N:N + S, 9] == 1) and x[N + S, 10] == 1
+    off, col = g["off"], g["col"]
+    deg = np.diff(off)
+    # each mesh node: its mesh neighbours plus its slab
+    mesh_deg = np.bincount(np.asarray(c.mesher.faces).ravel(), minlength=N)
+    assert np.array_equal(deg[:N], mesh_deg + 1)
+    # slab k holds the nodes with floor(x * S) == k, plus neighbour slabs and the global node
+    k = np.minimum((c.mesher.points[:, 0] * S).astype(int), S - 1)
+    cnt = np.bincount(k, minlength=S)
+    nb = np.full(S, 2)
+    nb[0] = nb[-1] = 1
+    assert np.array_equal(deg[N:N + S], cnt + nb + 1)
+    assert deg[N + S] == S
+    # transpose arrays: the weight and direction of each reverse edge
+    src = np.repeat(np.arange(g["n"]), deg)
+    key = {(a, b): i for i, (a, b) in enumerate(zip(src.tolist(), col.tolist()))}
+    for i in range(0, len(col), 97):
+        r = key[(col[i], src[i])]
+        assert g["wt"][i] == g["w"][r] and g["ext"][i] == g["ex"][r] and g["eyt"][i] == g["ey"][r]
+
+
 def test_twin_aggregate_matches_dense_adjacency():
-    c, (x, g) = _case()
+    c, g = _case()
     N = c.n_nodes
     p = c.mesher.points
     A = np.zeros((N, N))
