@@ -40,7 +40,8 @@ def test_virtual_nodes():
     x, g, N = sg.case_inputs(c)
     S = sg.SLABS
     assert N == c.n_nodes and g["n"] == N + S + 1 and x.shape == (N + S + 1, 16)
-    assert np.all(x[:N, 9:] == 0) and np.all(x[N:N + S, 9] == 1) and x[N + S, 10] == 1
+    assert np.all(x[:N, 9:11] == 0) and np.all(x[N:N + S, 9] == 1) and x[N + S, 10] == 1
+    assert np.all(x[:, 15] == 0)  # the last input column is padding
     off, col = g["off"], g["col"]
     deg = np.diff(off)
     # each mesh node: its mesh neighbours plus its slab
