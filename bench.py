@@ -14,6 +14,11 @@ elapsed / 1e6 (MLUPS). roofline.achieved = 108 B/LUP * cells / (elapsed/K).
 cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
 of the same mesh, on this host.
 
+batched: ``--batch-cases`` porous cases of the batched workload (5-40
+discs, 20k-200k cells, seeds 0..) in one handle: MLUPS of the batched step
+over all their cells, then a masked run to a 1e-6 residual with a step cap
+(many of these cases have no steady state; they stop as growing).
+
 steady_state (the second half of the metric): ``--steady-cases`` unseen
 porous-channel cases of the study distribution (seeds from 1,000,000; the
 surrogate was trained on seeds below 10,000) solved as one batch to a
@@ -109,6 +114,45 @@ def cpu_baseline(m, dyn, c, f, n, budget_s=12.0):
 WEIGHTS = os.path.join(ROOT, "fvdbm-jax_amd", "data", "surrogate_porous.npz")
 
 
+def _build_workload(seed):
+    fv = importlib.import_module("fvdbm-jax_amd")
+    return fv.porous.make_case(seed, dist=fv.porous.WORKLOAD)
+
+
+def batched(cases, steps=1000, masked_steps=20_000, check_every=500):
+    """Throughput of one batched launch over uneven porous cases (the
+    batched workload's distribution), then a masked run: every case steps
+    until its residual is below 1e-6, it diverges or grows 10x past its best,
+    or ``masked_steps`` have run."""
+    batch = importlib.import_module("fvdbm-jax_amd.batch")
+    ws = importlib.import_module("fvdbm-jax_amd.warmstart")
+    _lib = importlib.import_module("fvdbm-jax_amd._lib")
+    import ctypes as ct
+    b = batch.BatchSolver([c.env_tuple() for c in cases])
+    cells = int(sum(c.n_cells for c in cases))
+    info = _lib.Info()
+    _lib.check(_lib.lib().fvdbm_info_get(b._handle(), ct.byref(info)), "fvdbm_info_get")
+    b.step(100)
+    ms = ct.c_double()
+    _lib.check(_lib.lib().fvdbm_step_timed(b._handle(), int(steps), ct.byref(ms)), "fvdbm_step_timed")
+    per = ms.value / 1e3 / steps
+    mlups = cells / per / 1e6
+    b.reset_uniform()
+    b.set_active(np.ones(b.n, np.int32), reset_steps=True)
+    t0 = time.perf_counter()
+    st = b.run(masked_steps, check_every, 1e-6, 10.0)
+    wall = time.perf_counter() - t0
+    b.close()
+    counts = {ws_name: int((st["status"] == code).sum()) for code, ws_name in
+              ((ws.RUNNING, "running"), (ws.CONVERGED, "converged"), (ws.DIVERGED, "diverged"),
+               (ws.GROWING, "growing"))}
+    return dict(cases=len(cases), cells=cells, padded_cells=int(info.n_cells_padded), steps=steps,
+                us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
+                hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
+                masked=dict(max_steps=masked_steps, wall_s=round(wall, 2), **counts,
+                            steps_median=float(np.median(st["steps"]))))
+
+
 def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
     """Cold against warm-started batched solves of the same cases."""
     fv = importlib.import_module("fvdbm-jax_amd")
@@ -181,6 +225,8 @@ def main():
                     help="unseen porous cases for the cold/warm steady-state study (0: skip)")
     ap.add_argument("--steady-seed0", type=int, default=1_000_000)
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--batch-cases", type=int, default=32,
+                    help="porous cases of the batched workload for the batched throughput line (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); "
@@ -194,6 +240,13 @@ def main():
         t0 = time.time()
         steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases))
         log(f"steady-state study: {len(steady_cases)} cases meshed in {time.time() - t0:.1f}s")
+    batch_cases = None
+    if args.batch_cases > 0:
+        import multiprocessing as mp
+        t0 = time.time()
+        with mp.get_context("fork").Pool(16) as pool:
+            batch_cases = pool.map(_build_workload, range(args.batch_cases))
+        log(f"batched workload: {len(batch_cases)} cases meshed in {time.time() - t0:.1f}s")
 
     copy = None
     if args.copy_probe:
@@ -232,6 +285,11 @@ def main():
                 traffic=(tr["bytes_per_launch"] if tr else None))
     cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
     env.close()
+    bat = None
+    if batch_cases is not None:
+        t0 = time.time()
+        bat = batched(batch_cases)
+        log(f"batched {time.time() - t0:.1f}s: {bat}")
     steady = None
     if steady_cases is not None:
         t0 = time.time()
@@ -251,6 +309,7 @@ def main():
                    "residual_after": r},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": bat,
         "steady_state": steady,
         **({"copy_probe": copy} if copy else {}),
     }

@@ -338,6 +338,8 @@ struct StepArgs {
   const T *f0;  // the two pdf buffers (batched solves pick one per case)
   T *f1;
   int batch;
+  int bstep;                              // step index within the current batch interval
+  const int32_t *__restrict__ an_case;    // [nan] case of each active node (batched)
   int C, Cp, N, K, nbf, nan;
   int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs, max_halo;
   Lattice<T> k;
@@ -472,7 +474,9 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   if (a.batch) {  // the tile's case: skip it when frozen, else its parity and parameters
     const int cs = a.tile_case[tile];
     if (!a.case_active[cs]) return;  // uniform per block
-    const int cur = a.case_cur[cs];
+    // case_cur holds the parity at the start of the interval; the step
+    // index within it gives the current one (advanced after the interval)
+    const int cur = (a.case_cur[cs] + a.bstep) & 1;
     fsrc = cur ? a.f1 : a.f0;
     fdst = cur ? const_cast<T *>(a.f0) : a.f1;
     invtau = a.case_invtau[cs];
@@ -696,7 +700,9 @@ __global__ void k_nodes(StepArgs<T> a, const int32_t *__restrict__ an_gid, T *__
   const int an = blockIdx.x * blockDim.x + threadIdx.x;
   if (an >= a.nan) return;
   const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
-  const Buf fin(a.fsrc, Q * plane);
+  const T *src = a.fsrc;
+  if (a.batch) src = ((a.case_cur[a.an_case[an]] + a.bstep) & 1) ? a.f1 : a.f0;
+  const Buf fin(src, Q * plane);
   T out[Q], r, ux, uy;
   active_node(a, fin, plane, an, out, r, ux, uy);
   const int g = an_gid[an];
@@ -823,17 +829,18 @@ __global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ 
     dst[i] = src[i];
 }
 
-// Batched solves, after each step: active cases flip their buffer parity
-// and count the step.
+// Batched solves, after an interval of n steps (launched with step index
+// 0..n-1): active cases advance their buffer parity and step count by n.
 __global__ void k_batch_advance(int ncases, const int32_t *__restrict__ active,
-                                int32_t *__restrict__ cur, int64_t *__restrict__ steps) {
+                                int32_t *__restrict__ cur, int64_t *__restrict__ steps, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ncases || !active[k]) return;
-  cur[k] ^= 1;
-  steps[k] += 1;
+  cur[k] ^= (n & 1);
+  steps[k] += n;
 }
 
-// Batched solves, after a checked step: per case the residual of that step
+// Batched solves, after a checked interval of `every` steps: per case the
+// residual of its last step
 // from its tiles' partial sums (fixed order), then the case stops when it is
 // converged (r < tol), non-finite, or (growth > 0) r > growth * best r.
 // One block; n_active = number of cases still running.
@@ -842,11 +849,15 @@ __global__ void k_batch_check(int ncases, const int32_t *__restrict__ case_tile_
                               const double *__restrict__ partials, double tol, double growth,
                               int32_t *__restrict__ active, int32_t *__restrict__ status,
                               double *__restrict__ resid, double *__restrict__ best,
-                              int32_t *__restrict__ n_active) {
+                              int32_t *__restrict__ n_active, int32_t *__restrict__ cur,
+                              int64_t *__restrict__ steps, int every) {
   __shared__ int cnt[256];
   int mine = 0;
   for (int k = threadIdx.x; k < ncases; k += blockDim.x) {
     if (active[k]) {
+      // the interval's steps: parity and count, as k_batch_advance
+      cur[k] ^= (every & 1);
+      steps[k] += every;
       double d = 0.0, n = 0.0;
       for (int t = case_tile_off[k]; t < case_tile_off[k + 1]; ++t) {
         d += partials[2 * (size_t)t];
@@ -1027,6 +1038,10 @@ struct Handle {
        *w_nuy = nullptr, *w_pdf = nullptr;
   // batched solves
   int batch = 0, ncases = 1;
+  int bstep = 0;             // step index within the batch interval being launched
+  void *an_case = nullptr;   // [nan] case of each active node
+  hipGraphExec_t sexec = nullptr;  // batched: graph_steps steps + advance
+  hipGraph_t sgraph = nullptr;
   std::vector<int32_t> case_tile_off_host;
   std::vector<int64_t> case_cell_off;  // reference cell range of each case
   void *tile_case = nullptr, *case_tile_off = nullptr, *case_active = nullptr,
@@ -1052,6 +1067,8 @@ struct Handle {
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
     if (bexec) (void)hipGraphExecDestroy(bexec);
+    if (sexec) (void)hipGraphExecDestroy(sexec);
+    if (sgraph) (void)hipGraphDestroy(sgraph);
     if (bgraph) (void)hipGraphDestroy(bgraph);
     for (auto &b : bufs)
       if (b.p) (void)hipFree(b.p);
@@ -1112,6 +1129,8 @@ static StepArgs<T> make_args(const Handle &h, int src) {
   a.block_tile = static_cast<const int32_t *>(h.block_tile);
   a.tile_count = static_cast<const int32_t *>(h.tile_count);
   a.batch = h.batch;
+  a.bstep = h.bstep;
+  a.an_case = static_cast<const int32_t *>(h.an_case);
   a.tile_case = static_cast<const int32_t *>(h.tile_case);
   a.case_active = static_cast<const int32_t *>(h.case_active);
   a.case_cur = static_cast<const int32_t *>(h.case_cur);
@@ -1175,24 +1194,38 @@ static int launch_step(Handle &h, int src, bool resid) {
 }
 
 static int launch_step_any(Handle &h, int src, bool resid) {
-  int rc = h.precision == FVDBM_FP64 ? launch_step<double>(h, src, resid)
-                                     : launch_step<float>(h, src, resid);
-  if (rc || !h.batch) return rc;
-  // batched: active cases flip parity and count the step
+  return h.precision == FVDBM_FP64 ? launch_step<double>(h, src, resid)
+                                   : launch_step<float>(h, src, resid);
+}
+
+static int launch_batch_advance(Handle &h, int n) {
   hipLaunchKernelGGL(k_batch_advance, dim3((h.ncases + 255) / 256), dim3(256), 0, h.stream,
                      h.ncases, static_cast<const int32_t *>(h.case_active),
-                     static_cast<int32_t *>(h.case_cur), static_cast<int64_t *>(h.case_steps));
+                     static_cast<int32_t *>(h.case_cur), static_cast<int64_t *>(h.case_steps), n);
   HIP_TRY(hipGetLastError());
   return FVDBM_OK;
 }
 
-static int launch_batch_check(Handle &h, double tol, double growth) {
+// Batched: an interval of n steps (step index 0..n-1, the residual on the
+// last when asked) followed by the parity/step advance of active cases.
+static int launch_batch_interval(Handle &h, int n, bool resid_last) {
+  int rc = FVDBM_OK;
+  for (int j = 0; j < n && rc == FVDBM_OK; ++j) {
+    h.bstep = j;
+    rc = launch_step_any(h, 0, resid_last && j == n - 1);
+  }
+  h.bstep = 0;
+  return rc ? rc : launch_batch_advance(h, n);
+}
+
+static int launch_batch_check(Handle &h, double tol, double growth, int every) {
   hipLaunchKernelGGL(k_batch_check, dim3(1), dim3(256), 0, h.stream, h.ncases,
                      static_cast<const int32_t *>(h.case_tile_off),
                      static_cast<const double *>(h.partials), tol, growth,
                      static_cast<int32_t *>(h.case_active), static_cast<int32_t *>(h.case_status),
                      static_cast<double *>(h.case_resid), static_cast<double *>(h.case_best),
-                     static_cast<int32_t *>(h.n_active));
+                     static_cast<int32_t *>(h.n_active), static_cast<int32_t *>(h.case_cur),
+                     static_cast<int64_t *>(h.case_steps), every);
   HIP_TRY(hipGetLastError());
   return FVDBM_OK;
 }
@@ -1212,9 +1245,12 @@ static int ensure_batch_graph(Handle &h, int every, double tol, double growth) {
   }
   HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
   int rc = FVDBM_OK;
-  for (int s = 0; s < every - 1 && rc == FVDBM_OK; ++s) rc = launch_step_any(h, 0, false);
-  if (!rc) rc = launch_step_any(h, 0, true);
-  if (!rc) rc = launch_batch_check(h, tol, growth);
+  for (int s = 0; s < every && rc == FVDBM_OK; ++s) {
+    h.bstep = s;
+    rc = launch_step_any(h, 0, s == every - 1);
+  }
+  h.bstep = 0;
+  if (!rc) rc = launch_batch_check(h, tol, growth, every);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(h.stream, &g);
   if (rc) {
@@ -1251,8 +1287,48 @@ static int ensure_graph(Handle &h, int parity) {
   return FVDBM_OK;
 }
 
+// Batched: a graph of graph_steps steps and the advance, replayed.
+static int ensure_batch_step_graph(Handle &h) {
+  if (h.sexec) return FVDBM_OK;
+  HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_batch_interval(h, h.graph_steps, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h.sgraph = g;
+  HIP_TRY(hipGraphInstantiate(&h.sexec, g, nullptr, nullptr, 0));
+  return FVDBM_OK;
+}
+
+static int run_batch_steps(Handle &h, int64_t n, bool resid_last) {
+  int64_t rest = n;
+  if (h.graph_steps > 0) {
+    while (rest - (resid_last ? 1 : 0) >= h.graph_steps) {
+      int rc = ensure_batch_step_graph(h);
+      if (rc) return rc;
+      HIP_TRY(hipGraphLaunch(h.sexec, h.stream));
+      rest -= h.graph_steps;
+    }
+  }
+  while (rest > 0) {
+    const int chunk = (int)std::min<int64_t>(rest, 1024);
+    int rc = launch_batch_interval(h, chunk, resid_last && chunk == rest);
+    if (rc) return rc;
+    rest -= chunk;
+  }
+  h.steps += n;
+  h.has_prev = true;
+  h.observed = false;
+  return FVDBM_OK;
+}
+
 static int run_steps(Handle &h, int64_t n, bool resid_last) {
   if (n <= 0) return FVDBM_OK;
+  if (h.batch) return run_batch_steps(h, n, resid_last);
   int64_t plain = resid_last ? n - 1 : n;
   if (h.graph_steps > 0 && !h.batch) {
     while (plain >= h.graph_steps) {
@@ -1368,6 +1444,15 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
     if (!rc) rc = h.upload(&h.case_best, inf);
     if (!rc) rc = h.upload(&h.case_invtau, it);
     if (!rc) rc = h.upload(&h.case_dt, dtv);
+    if (!rc) {
+      // each active node's case: that of its first listed cell
+      std::vector<int32_t> anc(std::max<size_t>(L.an_gid.size(), 1), 0);
+      for (size_t i = 0; i < L.an_gid.size(); ++i) {
+        const int c0 = m.node_cells[(size_t)L.an_gid[i] * m.k_max];
+        anc[i] = c0 >= 0 ? cell_case[c0] : 0;
+      }
+      rc = h.upload(&h.an_case, anc);
+    }
     if (!rc) rc = h.alloc(&h.n_active, sizeof(int32_t));
   }
   if (!rc) rc = h.upload(&h.halo_off, L.halo_off);
@@ -2103,7 +2188,7 @@ int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms) {
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   // build graphs outside the timed region
   if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
-    int rc = fvdbm::ensure_graph(*h, h->cur);
+    int rc = h->batch ? fvdbm::ensure_batch_step_graph(*h) : fvdbm::ensure_graph(*h, h->cur);
     if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));

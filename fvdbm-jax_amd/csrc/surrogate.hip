@@ -36,7 +36,7 @@
 namespace fvdbm_nn {
 
 constexpr int HID = 64;
-constexpr int CHUNK = 2048;  // rows per partial product in wgrad / colsum
+constexpr int CHUNK = 256;  // rows per partial product in wgrad / colsum
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -274,14 +274,25 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
   }
 }
 
-// partial[c][n] = sum of column n over the rows of chunk c (float32, row order)
-__global__ void k_colsum(const float *__restrict__ A, int M, int N, float *__restrict__ partial) {
-  const int n = threadIdx.x;
-  if (n >= N) return;
+// partial[c][n] = sum of column n over the rows of chunk c: 256 / N row
+// groups per column each sum their rows in order, then the groups are added
+// in group order (fixed order, deterministic)
+__global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ A, int M, int N,
+                                                float *__restrict__ partial) {
+  __shared__ float red[256];
+  const int groups = 256 / N;
+  const int n = threadIdx.x % N, grp = threadIdx.x / N;
   const int r0 = blockIdx.x * CHUNK, r1 = min(M, r0 + CHUNK);
   float s = 0.f;
-  for (int r = r0; r < r1; ++r) s += A[(size_t)r * N + n];
-  partial[(size_t)blockIdx.x * N + n] = s;
+  if (grp < groups)
+    for (int r = r0 + grp; r < r1; r += groups) s += A[(size_t)r * N + n];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < N) {
+    float t = 0.f;
+    for (int g = 0; g < groups; ++g) t += red[g * N + threadIdx.x];
+    partial[(size_t)blockIdx.x * N + threadIdx.x] = t;
+  }
 }
 
 __global__ void k_zero(float *__restrict__ p, int n) {

@@ -48,24 +48,33 @@ def test_batch_of_one_is_single(gpu):
     assert np.array_equal(b.get_pdf(), single)
 
 
-def test_run_residuals_and_freezing(gpu):
+@pytest.mark.parametrize("variant", [0, 2])
+def test_run_residuals_and_freezing(gpu, variant):
+    """variant 2 computes boundary nodes in their own kernel, which must
+    also read each case's own buffer once the cases' parities differ."""
     cs = _cases(3)
-    b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs])
+    b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs], variant=variant)
     st = b.run(max_steps=200, check_every=50, tol=0.0)  # tol 0: nobody converges
     assert b.steps_run == 200 and np.all(st["steps"] == 200) and st["active"].all()
     for k, (m, dyn, c, f, n) in enumerate(cs):
         single, r = _single(m, dyn, c, f, n, 200, resid=True)
         assert np.array_equal(b.case_pdf(k), single)
         assert st["residual"][k] == pytest.approx(r, rel=1e-12)
-    # stop case 1; the others go on, case 1 stays bitwise frozen
+    # stop case 1; the others go on an odd number of steps (their buffer
+    # parity now differs from case 1's), case 1 stays bitwise frozen
     frozen = b.case_pdf(1).copy()
     b.set_active([1, 0, 1])
-    st = b.run(max_steps=100, check_every=50, tol=0.0)
+    st = b.run(max_steps=75, check_every=25, tol=0.0)
     assert np.array_equal(b.case_pdf(1), frozen)
-    assert st["steps"].tolist() == [300, 200, 300]
+    assert st["steps"].tolist() == [275, 200, 275]
     assert st["status"][1] == 4  # stopped
-    single, _ = _single(*cs[0], 300)
+    single, _ = _single(*cs[0], 275)
     assert np.array_equal(b.case_pdf(0), single)
+    b.step(33)  # plain steps (graph replays plus a remainder)
+    single, _ = _single(*cs[2], 308)
+    assert np.array_equal(b.case_pdf(2), single)
+    assert np.array_equal(b.case_pdf(1), frozen)
+    assert b.status()["steps"].tolist() == [308, 200, 308]
 
 
 def test_run_until_converged(gpu):
