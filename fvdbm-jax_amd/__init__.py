@@ -14,7 +14,7 @@ Environment.step() path.
 """
 import sys as _sys
 
-from . import meshgen, porous
+from . import autotune, meshgen, porous
 from .containers import Cells, CustomArray, Faces, Nodes
 from .dynamics import D2Q9, Dynamics
 from .environment import Environment
@@ -23,4 +23,4 @@ from .mesher import Mesher
 _sys.modules.setdefault("fvdbm_jax_amd", _sys.modules[__name__])
 
 __all__ = ["Cells", "CustomArray", "D2Q9", "Dynamics", "Environment", "Faces",
-           "Mesher", "Nodes", "meshgen", "porous"]
+           "Mesher", "Nodes", "autotune", "meshgen", "porous"]
