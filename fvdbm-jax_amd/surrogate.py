@@ -136,28 +136,41 @@ SECTION_SHIFT = np.array([0.95, 1.06, 0.5, 0.26], np.float32)
 SECTION_SCALE = np.array([0.085, 0.116, 0.31, 0.106], np.float32)
 
 
-def case_inputs(case, slabs=SLABS, section=True):
-    """Scaled and padded node inputs [N + slabs + 1, 16] of a porous.Case
-    (mesh nodes first, then slab nodes, then the global node), its graph,
-    and the number of mesh nodes. Columns 0-8: the node features, 9-10:
-    virtual-node flags, 11-14 (with ``section``): porous.section_features."""
+def case_inputs(case, grid=(SLABS, 1), section=True):
+    """Scaled and padded node inputs [N + GX*GY + 1, 16] of a porous.Case
+    (mesh nodes first, then the virtual grid nodes in row-major order, then
+    the global node), its graph, and the number of mesh nodes.
+
+    The virtual grid covers the channel with GX x GY cells: each mesh node
+    is linked both ways to the node of its grid cell, grid nodes to their
+    8 neighbours (with the unit direction between centres), and the global
+    node to every grid node. GY = 1 gives bands of x ("slabs"). Columns
+    0-8: the node features, 9-10: virtual-node flags (grid, global), 11-14
+    (with ``section``): porous.section_features."""
     f = porous.node_features(case)
     pts = np.asarray(case.mesher.points, np.float64)
     N = f.shape[0]
-    S = slabs
+    GX, GY = grid
+    S = GX * GY
     n = N + S + 1
     x = np.zeros((n, IN_PAD), np.float32)
     x[:N, :f.shape[1]] = (f - FEATURE_SHIFT) / FEATURE_SCALE
+    gi, gj = np.meshgrid(np.arange(GX), np.arange(GY), indexing="xy")
+    gi, gj = gi.ravel(), gj.ravel()  # row-major: index = j * GX + i
+    cx = (gi + 0.5) / GX
+    cy = (gj + 0.5) / GY * porous.HEIGHT
     if section:
-        vx = np.concatenate([pts[:, 0], (np.arange(S) + 0.5) / S, [0.5]])
+        vx = np.concatenate([pts[:, 0], cx, [0.5]])
         sf = porous.section_features(np.stack([vx, np.zeros_like(vx)], 1), case.discs)
         x[:, 11:15] = (sf - SECTION_SHIFT) / SECTION_SCALE
-    xc = (np.arange(S) + 0.5) / S
     virt = np.zeros((S + 1, f.shape[1]), np.float32)
-    virt[:S, 0] = xc
+    virt[:S, 0] = cx
     virt[S, 0] = 0.5
     virt[:, 1] = 0.25
     virt[:, 2] = FEATURE_SHIFT[2]
+    if GY > 1:
+        virt[:S, 1] = cy
+        virt[:S, 2] = porous.signed_distance(np.stack([cx, cy], 1), case.discs)
     virt[:, 7] = case.u_in
     virt[:, 8] = case.dynamics.tau
     x[N:, :f.shape[1]] = (virt - FEATURE_SHIFT) / FEATURE_SCALE
@@ -171,20 +184,31 @@ def case_inputs(case, slabs=SLABS, section=True):
     dst = [e[:, 1], e[:, 0]]
     exs = [d[:, 0] / L, -d[:, 0] / L]
     eys = [d[:, 1] / L, -d[:, 1] / L]
-    # node <-> its slab (no direction)
-    slab = np.minimum((pts[:, 0] * S).astype(np.int64), S - 1) + N
+    # mesh node <-> its grid node (no direction)
+    ci = np.minimum((pts[:, 0] * GX).astype(np.int64), GX - 1)
+    cj = np.minimum((pts[:, 1] / porous.HEIGHT * GY).astype(np.int64), GY - 1)
+    cell = cj * GX + ci + N
     nodes = np.arange(N)
     z = np.zeros(N)
-    src += [nodes, slab]
-    dst += [slab, nodes]
+    src += [nodes, cell]
+    dst += [cell, nodes]
     exs += [z, z]
     eys += [z, z]
-    # slab <-> next slab (along x), slab <-> global
-    a = np.arange(S - 1) + N
-    src += [a, a + 1]
-    dst += [a + 1, a]
-    exs += [np.ones(S - 1), -np.ones(S - 1)]
-    eys += [np.zeros(S - 1), np.zeros(S - 1)]
+    # grid node -> each of its 8 neighbours (both directions arise in turn)
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            if di == 0 and dj == 0:
+                continue
+            ok = (gi + di >= 0) & (gi + di < GX) & (gj + dj >= 0) & (gj + dj < GY)
+            a = (gj * GX + gi)[ok] + N
+            b = ((gj + dj) * GX + gi + di)[ok] + N
+            dx, dy = di / GX, dj / GY * porous.HEIGHT
+            L2 = np.hypot(dx, dy)
+            src.append(a)
+            dst.append(b)
+            exs.append(np.full(len(a), dx / L2))
+            eys.append(np.full(len(a), dy / L2))
+    # grid <-> global
     sl = np.arange(S) + N
     gnode = np.full(S, N + S)
     src += [sl, gnode]
@@ -334,9 +358,12 @@ def _block_names(layers):
 class MeshNet:
     """The surrogate network; parameters are torch tensors on the GPU."""
 
-    def __init__(self, params=None, layers=LAYERS, seed=0, target_scale=(1.0, 1.0, 1.0)):
+    def __init__(self, params=None, layers=LAYERS, seed=0, target_scale=(1.0, 1.0, 1.0),
+                 grid=(SLABS, 1)):
         torch = _torch()
         self.layers = layers
+        self.grid = tuple(int(v) for v in (params["grid"] if params is not None and "grid" in params
+                                           else grid))
         p = init_params(seed, layers) if params is None else params
         self.params = {k: torch.tensor(np.asarray(v, np.float32), device="cuda", requires_grad=True)
                        for k, v in p.items() if "." in k}
@@ -363,6 +390,7 @@ class MeshNet:
         out = {k: v.detach().cpu().numpy() for k, v in self.params.items()}
         out["target_scale"] = self.target_scale
         out["layers"] = np.int32(self.layers)
+        out["grid"] = np.array(self.grid, np.int32)
         return out
 
     def save(self, path):
@@ -378,7 +406,7 @@ class MeshNet:
         """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
         with ``project``, u_x passes through ``project_mass_flux``."""
         torch = _torch()
-        ins = [case_inputs(c) for c in cases]
+        ins = [case_inputs(c, grid=self.grid) for c in cases]
         g = graph_to_device(concat_graphs([i[1] for i in ins]))
         x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
         with torch.no_grad():

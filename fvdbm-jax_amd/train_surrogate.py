@@ -89,6 +89,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--grid", default="32x16", help="virtual grid GXxGY (GY = 1: bands of x)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -105,6 +106,7 @@ def main(argv=None):
     ok = [i for i, f in enumerate(fields) if f["status"] == warmstart.CONVERGED]
     log(f"solved in {t_solve:.1f} s: {len(ok)}/{len(cases)} converged")
     val_idx, train_idx = ok[-a.val:], ok[:-a.val]
+    grid = tuple(int(v) for v in a.grid.split("x"))
     t_all = np.concatenate([surrogate.targets(fields[i]["rho"], fields[i]["vel"], cases[i].u_in, (1, 1, 1))
                             for i in train_idx])
     scale = t_all.astype(np.float64).std(axis=0)
@@ -113,13 +115,13 @@ def main(argv=None):
     def samples(idx):
         out = []
         for i in idx:
-            x, g, nm = surrogate.case_inputs(cases[i])
+            x, g, nm = surrogate.case_inputs(cases[i], grid=grid)
             y = surrogate.targets(fields[i]["rho"], fields[i]["vel"], cases[i].u_in, scale, g["n"] - nm)
             out.append(surrogate.device_sample(x, g, y, nm))
         return out
 
     tr, va = samples(train_idx), samples(val_idx)
-    net = surrogate.MeshNet(seed=0, target_scale=scale)
+    net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid)
     t2 = time.perf_counter()
     hist = surrogate.train(net, tr, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, log=log)
     t_train = time.perf_counter() - t2
@@ -129,7 +131,7 @@ def main(argv=None):
     state = net.state()
     record = dict(cases=a.cases, seed0=a.seed0, converged=len(ok), train=len(train_idx), val=len(val_idx),
                   tol=a.tol, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, loss=hist,
-                  slabs=surrogate.SLABS,
+                  grid=list(grid),
                   val_rel_l2_median=np.median(err, axis=0).tolist(), val_rel_l2_max=err.max(axis=0).tolist(),
                   solve_s=t_solve, train_s=t_train,
                   solve_steps_median=float(np.median([fields[i]["steps"] for i in ok])))
