@@ -15,9 +15,10 @@ cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
 of the same mesh, on this host.
 
 batched: ``--batch-cases`` porous cases of the batched workload (5-40
-discs, 20k-200k cells, seeds 0..) in one handle: MLUPS of the batched step
-over all their cells, then a masked run to a 1e-6 residual with a step cap
-(many of these cases have no steady state; they stop as growing).
+discs, 20k-200k cells, seeds 0..) in one handle: the autotuner picks tile
+size and kernel variant (every candidate checked bitwise), then MLUPS of the
+batched step over all their cells, then a masked run to a 1e-6 residual
+with a step cap (many of these cases have no steady state).
 
 steady_state (the second half of the metric): ``--steady-cases`` unseen
 porous-channel cases of the study distribution (seeds from 1,000,000; the
@@ -128,7 +129,14 @@ def batched(cases, steps=1000, masked_steps=20_000, check_every=500):
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     _lib = importlib.import_module("fvdbm-jax_amd._lib")
     import ctypes as ct
-    b = batch.BatchSolver([c.env_tuple() for c in cases])
+    at = importlib.import_module("fvdbm-jax_amd.autotune")
+    triples = [c.env_tuple() for c in cases]
+    tuned = at.tune_batch(triples, cands=at.candidates(blocks=(128, 256, 512), variants=(0, 2, 4),
+                                                       reorders=(4,)),
+                          steps=200, cache=None, log=log)
+    best = tuned["best"]
+    default = [r for r in tuned["results"] if (r["block"], r["variant"]) == (256, 0)]
+    b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
     cells = int(sum(c.n_cells for c in cases))
     info = _lib.Info()
     _lib.check(_lib.lib().fvdbm_info_get(b._handle(), ct.byref(info)), "fvdbm_info_get")
@@ -147,6 +155,8 @@ def batched(cases, steps=1000, masked_steps=20_000, check_every=500):
               ((ws.RUNNING, "running"), (ws.CONVERGED, "converged"), (ws.DIVERGED, "diverged"),
                (ws.GROWING, "growing"))}
     return dict(cases=len(cases), cells=cells, padded_cells=int(info.n_cells_padded), steps=steps,
+                tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"]),
+                default_us_per_step=default[0]["us_per_step"] if default else None,
                 us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
                 hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
                 masked=dict(max_steps=masked_steps, wall_s=round(wall, 2), **counts,
@@ -284,6 +294,21 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=(tr["bytes_per_launch"] if tr else None))
     cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
+    fp64 = None
+    if args.precision == "fp32":
+        # the FP32 error of this run: the same steps with the same kernels in FP64
+        total = args.warmup + args.steps + 1
+        r32, u32 = env.cell_fields()
+        e64 = fv.Environment(c, f, n, precision="fp64", variant=args.variant, block=args.block,
+                             reorder=args.reorder)
+        e64.init()
+        e64.step(total)
+        r64, u64 = e64.cell_fields()
+        e64.close()
+        du = np.linalg.norm(u32.astype(np.float64) - u64) / np.linalg.norm(u64)
+        dr = np.linalg.norm(r32.astype(np.float64) - r64) / np.linalg.norm(r64)
+        fp64 = dict(steps=total, rel_l2_u=float(du), rel_l2_rho=float(dr))
+        log(f"fp32 vs fp64 after {total} steps: u {du:.3e}, rho {dr:.3e} (relative L2)")
     env.close()
     bat = None
     if batch_cases is not None:
@@ -308,6 +333,7 @@ def main():
                    "kernel": "k_step (fused cell step)", "bytes_per_lup": bpl,
                    "residual_after": r},
         "roofline": roof,
+        "fp32_vs_fp64": fp64,
         "cpu_baseline": cpu,
         "batched": bat,
         "steady_state": steady,
