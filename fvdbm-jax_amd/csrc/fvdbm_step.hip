@@ -235,41 +235,43 @@ __device__ __forceinline__ void Buf::st<double>(double v, uint32_t voff, uint32_
 // bytes (FP32) or 80 bytes (FP64), so a neighbour is 3 (5) aligned reads.
 template <typename T>
 struct LdsCell;
+// Records of Q = 9 values with an odd stride in 32-bit words (9 for FP32;
+// FP64 records are 18 words whose even start banks, with each value's
+// second word, cover all 32 banks), so the records that 32 lanes read at
+// scattered slots spread over all LDS banks. A 12-word stride (16-byte
+// aligned records) used only a quarter of the banks: 4-way conflicts.
 template <>
 struct LdsCell<float> {
-  static constexpr int W = 12;  // floats per cell record
+  static constexpr int W = 9;  // floats per cell record
   __device__ static __forceinline__ void put(float *base, int slot, const float (&f)[Q]) {
-    float4 *d = reinterpret_cast<float4 *>(base + slot * W);
-    d[0] = make_float4(f[0], f[1], f[2], f[3]);
-    d[1] = make_float4(f[4], f[5], f[6], f[7]);
-    d[2] = make_float4(f[8], 0.f, 0.f, 0.f);
+    float *d = base + slot * W;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) d[q] = f[q];
   }
   __device__ static __forceinline__ void get(const float *base, int slot, float (&f)[Q]) {
-    const float4 *d = reinterpret_cast<const float4 *>(base + slot * W);
-    const float4 a = d[0], b = d[1], c = d[2];
-    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
-    f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
-    f[8] = c.x;
+    const float *d = base + slot * W;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = d[q];
   }
 };
 template <>
 struct LdsCell<double> {
-  static constexpr int W = 10;  // doubles per cell record
+  static constexpr int W = 9;  // doubles per cell record
   __device__ static __forceinline__ void put(double *base, int slot, const double (&f)[Q]) {
-    double2 *d = reinterpret_cast<double2 *>(base + slot * W);
-    d[0] = make_double2(f[0], f[1]);
-    d[1] = make_double2(f[2], f[3]);
-    d[2] = make_double2(f[4], f[5]);
-    d[3] = make_double2(f[6], f[7]);
-    d[4] = make_double2(f[8], 0.0);
+    double *d = base + slot * W;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) d[q] = f[q];
   }
   __device__ static __forceinline__ void get(const double *base, int slot, double (&f)[Q]) {
-    const double2 *d = reinterpret_cast<const double2 *>(base + slot * W);
-    const double2 a = d[0], b = d[1], c = d[2], e = d[3], g = d[4];
-    f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = c.x;
-    f[5] = c.y; f[6] = e.x; f[7] = e.y; f[8] = g.x;
+    const double *d = base + slot * W;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = d[q];
   }
 };
+
+// Stride of a (node, cell) pair's contribution record in the node phase:
+// 12 values (rho w, ux w, uy w, 9 neq w) padded to an odd 13 words.
+constexpr int kPairW = 13;
 
 // One face of one cell: neighbour code, face normal, sign * length.
 // FP32: one 16-byte record; FP64: two 16-byte records (code, pad, nx | ny, sL).
@@ -500,7 +502,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
       T fc[Q], r, ux, uy;
       load_cell(fin, plane, cj < 0 ? -2 - cj : cj, fc);
       macro(fc, r, ux, uy);
-      T *o = con + 12 * p;
+      T *o = con + kPairW * p;
       o[0] = r * wj;
       o[1] = ux * wj;
       o[2] = uy * wj;
@@ -528,7 +530,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[q] = T(0);
         for (int p = p0; p < p1; ++p) {
-          const T *o = con + 12 * p;
+          const T *o = con + kPairW * p;
           if (type == 1) {
             n0 += o[0];
           } else {
@@ -566,7 +568,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   const int base = tile * bs;
   const int c = base + threadIdx.x;
   // 16-byte aligned start of the stage (after node table and contributions)
-  const int pre = (Q * a.max_tile_nodes + 12 * a.max_tile_pairs + 7) & ~7;
+  const int pre = (Q * a.max_tile_nodes + kPairW * a.max_tile_pairs + 7) & ~7;
   T *shf = static_cast<T *>(__builtin_assume_aligned(sh + pre, 16));
   T f[Q];
   FaceRec<T> fr[3] = {};
@@ -1160,8 +1162,8 @@ static StepArgs<T> make_args(const Handle &h, int src) {
 // dynamic LDS of k_step: node table, pair contributions, pdf stage, face table
 static size_t lds_bytes(const Handle &h, size_t es) {
   const bool stage = (h.variant & 4) == 0;
-  size_t n = ((size_t)Q * h.max_tile_nodes + 12 * (size_t)h.max_tile_pairs + 7) & ~(size_t)7;
-  if (stage) n += (size_t)(es == 4 ? 12 : 10) * (h.block + h.max_halo);
+  size_t n = ((size_t)Q * h.max_tile_nodes + kPairW * (size_t)h.max_tile_pairs + 7) & ~(size_t)7;
+  if (stage) n += (size_t)9 * (h.block + h.max_halo);
   return n * es;
 }
 
@@ -1382,7 +1384,7 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
   h.max_tile_pairs = L.max_tile_pairs;
   // LDS per block is capped at 64 KiB: node tables beyond 40 KiB use the
   // two-kernel variant, and the pdf staging is dropped when it does not fit
-  if (((size_t)Q * L.max_tile_nodes + 12 * (size_t)L.max_tile_pairs) * sizeof(T) > 40960) {
+  if (((size_t)Q * L.max_tile_nodes + kPairW * (size_t)L.max_tile_pairs) * sizeof(T) > 40960) {
     h.variant |= 2;
     h.max_tile_nodes = 0;
     h.max_tile_pairs = 0;
