@@ -90,7 +90,8 @@ typedef struct fvdbm_params {
                          with ghost-face cells first, 2 RCM, 3 Hilbert curve of the
                          centroids with ghost-face cells first, 4 Hilbert curve
                          (3 and 4 need cell_xy) */
-  int32_t variant;    /* step kernel variant, 0 = default (see fvdbm_variant_count) */
+  int32_t variant;    /* step kernel variant bits, 0 = default: 1 no XCD-aware tile order,
+                         2 boundary nodes in their own kernel, 4 no LDS staging */
   int32_t block;      /* threads per block (64..1024, multiple of 64), 0 = default */
   int32_t graph_steps;/* steps per captured hipGraph, 0 = default, <0 = no graphs */
 } fvdbm_params;
