@@ -231,7 +231,7 @@ def main():
                     help="also time a 256 MiB device copy (counter calibration)")
     ap.add_argument("--mesh-cache", default=None,
                     help="directory to keep the generated mesh between runs")
-    ap.add_argument("--steady-cases", type=int, default=256,
+    ap.add_argument("--steady-cases", type=int, default=1024,
                     help="unseen porous cases for the cold/warm steady-state study (0: skip)")
     ap.add_argument("--steady-seed0", type=int, default=1_000_000)
     ap.add_argument("--weights", default=WEIGHTS)

@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import math
+import time
 
 import numpy as np
 
@@ -108,28 +109,49 @@ def graph_to_device(g, device="cuda"):
 SLABS = 32
 
 
-def graph_from_edges(n, src, dst, ex, ey):
+def graph_from_edges(n, src, dst, ex, ey, paired=False):
     """CSR graph from directed edges (any order): rows sorted by (source,
     target); weights 1/deg(source); transpose arrays for the backward pass
-    from the reverse edge (which must exist)."""
+    from the reverse edge (which must exist). With ``paired``, edges 2k and
+    2k + 1 are each other's reverse, which spares the search for it."""
     src = np.asarray(src, np.int64)
     dst = np.asarray(dst, np.int64)
-    order = np.lexsort((dst, src))
-    src, dst = src[order], dst[order]
-    ex = np.asarray(ex, np.float64)[order]
-    ey = np.asarray(ey, np.float64)[order]
+    ex = np.asarray(ex, np.float64)
+    ey = np.asarray(ey, np.float64)
+    key = src * n + dst
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    if paired:
+        pos = np.empty_like(order)
+        pos[order] = np.arange(len(order))
+        rev = pos[order ^ 1]
+    else:
+        rkey = dst[order] * n + src[order]
+        rev = np.searchsorted(key, rkey)
+        if not np.array_equal(key[np.minimum(rev, len(key) - 1)], rkey):
+            raise ValueError("graph edges must come in both directions")
+    src, dst, ex, ey = src[order], dst[order], ex[order], ey[order]
     deg = np.bincount(src, minlength=n)
     off = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
-    # the reverse edge of (s, d) is (d, s): find it by key
-    key = src * n + dst
-    rkey = dst * n + src
-    rev = np.searchsorted(key, rkey)
-    if not np.array_equal(key[rev], rkey):
-        raise ValueError("graph edges must come in both directions")
     return dict(off=off, col=dst.astype(np.int32), w=(1.0 / deg[src]).astype(np.float32),
                 ex=ex.astype(np.float32), ey=ey.astype(np.float32),
                 wt=(1.0 / deg[dst]).astype(np.float32), ext=ex[rev].astype(np.float32),
                 eyt=ey[rev].astype(np.float32), n=int(n))
+
+
+def _interleave(u, v, ex, ey):
+    """Directed pairs (u -> v, v -> u) of undirected edges, as arrays in
+    the order 2k, 2k + 1."""
+    m = len(u)
+    src = np.empty(2 * m, np.int64)
+    dst = np.empty(2 * m, np.int64)
+    dx = np.empty(2 * m)
+    dy = np.empty(2 * m)
+    src[0::2], src[1::2] = u, v
+    dst[0::2], dst[1::2] = v, u
+    dx[0::2], dx[1::2] = ex, -np.asarray(ex, np.float64)
+    dy[0::2], dy[1::2] = ey, -np.asarray(ey, np.float64)
+    return src, dst, dx, dy
 
 
 SECTION_SHIFT = np.array([0.95, 1.06, 0.5, 0.26], np.float32)
@@ -176,47 +198,27 @@ def case_inputs(case, grid=(SLABS, 1), section=True):
     x[N:, :f.shape[1]] = (virt - FEATURE_SHIFT) / FEATURE_SCALE
     x[N:N + S, 9] = 1.0
     x[N + S, 10] = 1.0
-    # mesh edges
+    # undirected edges: mesh faces, mesh node - its grid cell, grid cell -
+    # 4 of its 8 neighbours (the other 4 are the reverses), grid - global
+    us, vs, exs, eys = [], [], [], []
     e = porous.mesh_edges(case).astype(np.int64)
     d = pts[e[:, 1]] - pts[e[:, 0]]
     L = np.hypot(d[:, 0], d[:, 1])
-    src = [e[:, 0], e[:, 1]]
-    dst = [e[:, 1], e[:, 0]]
-    exs = [d[:, 0] / L, -d[:, 0] / L]
-    eys = [d[:, 1] / L, -d[:, 1] / L]
-    # mesh node <-> its grid node (no direction)
+    us.append(e[:, 0]); vs.append(e[:, 1]); exs.append(d[:, 0] / L); eys.append(d[:, 1] / L)
     ci = np.minimum((pts[:, 0] * GX).astype(np.int64), GX - 1)
     cj = np.minimum((pts[:, 1] / porous.HEIGHT * GY).astype(np.int64), GY - 1)
-    cell = cj * GX + ci + N
-    nodes = np.arange(N)
-    z = np.zeros(N)
-    src += [nodes, cell]
-    dst += [cell, nodes]
-    exs += [z, z]
-    eys += [z, z]
-    # grid node -> each of its 8 neighbours (both directions arise in turn)
-    for di in (-1, 0, 1):
-        for dj in (-1, 0, 1):
-            if di == 0 and dj == 0:
-                continue
-            ok = (gi + di >= 0) & (gi + di < GX) & (gj + dj >= 0) & (gj + dj < GY)
-            a = (gj * GX + gi)[ok] + N
-            b = ((gj + dj) * GX + gi + di)[ok] + N
-            dx, dy = di / GX, dj / GY * porous.HEIGHT
-            L2 = np.hypot(dx, dy)
-            src.append(a)
-            dst.append(b)
-            exs.append(np.full(len(a), dx / L2))
-            eys.append(np.full(len(a), dy / L2))
-    # grid <-> global
-    sl = np.arange(S) + N
-    gnode = np.full(S, N + S)
-    src += [sl, gnode]
-    dst += [gnode, sl]
-    exs += [np.zeros(S), np.zeros(S)]
-    eys += [np.zeros(S), np.zeros(S)]
-    g = graph_from_edges(n, np.concatenate(src), np.concatenate(dst), np.concatenate(exs),
-                         np.concatenate(eys))
+    us.append(np.arange(N)); vs.append(cj * GX + ci + N); exs.append(np.zeros(N)); eys.append(np.zeros(N))
+    for di, dj in ((1, 0), (1, 1), (0, 1), (-1, 1)):
+        ok = (gi + di >= 0) & (gi + di < GX) & (gj + dj >= 0) & (gj + dj < GY)
+        a = (gj * GX + gi)[ok] + N
+        b = ((gj + dj) * GX + gi + di)[ok] + N
+        dx, dy = di / GX, dj / GY * porous.HEIGHT
+        L2 = np.hypot(dx, dy)
+        us.append(a); vs.append(b); exs.append(np.full(len(a), dx / L2)); eys.append(np.full(len(a), dy / L2))
+    us.append(np.arange(S) + N); vs.append(np.full(S, N + S)); exs.append(np.zeros(S)); eys.append(np.zeros(S))
+    src, dst, dx, dy = _interleave(np.concatenate(us), np.concatenate(vs), np.concatenate(exs),
+                                   np.concatenate(eys))
+    g = graph_from_edges(n, src, dst, dx, dy, paired=True)
     return x, g, N
 
 
@@ -406,7 +408,9 @@ class MeshNet:
         """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
         with ``project``, u_x passes through ``project_mass_flux``."""
         torch = _torch()
-        ins = [case_inputs(c, grid=self.grid) for c in cases]
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=8) as pool:  # NumPy releases the GIL in the sorts
+            ins = list(pool.map(lambda c: case_inputs(c, grid=self.grid), cases))
         g = graph_to_device(concat_graphs([i[1] for i in ins]))
         x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
         with torch.no_grad():
@@ -501,7 +505,7 @@ def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final
     for ep in range(epochs):
         order = rng.permutation(len(samples))
         losses = []
-        t_ep = __import__("time").perf_counter()
+        t_ep = time.perf_counter()
         for b0 in range(0, len(order), batch):
             for grp in opt.param_groups:
                 grp["lr"] = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / total))
@@ -515,7 +519,7 @@ def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final
         hist.append(float(torch.stack(losses).mean()))
         if log:
             log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e} "
-                f"({__import__('time').perf_counter() - t_ep:.1f} s)")
+                f"({time.perf_counter() - t_ep:.1f} s)")
     return hist
 
 
