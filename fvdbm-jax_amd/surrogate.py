@@ -404,24 +404,30 @@ class MeshNet:
         params = {k: z[k] for k in z.files if k != "record"}
         return cls(params, layers=int(z["layers"]))
 
-    def predict(self, cases, project=False):
+    def predict(self, cases, project=False, chunk=64):
         """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
-        with ``project``, u_x passes through ``project_mass_flux``."""
+        with ``project``, u_x passes through ``project_mass_flux``. Cases
+        go through the network ``chunk`` at a time; their inputs are
+        prepared by a thread pool (NumPy releases the GIL in the sorts)
+        while the previous chunk runs on the GPU."""
         torch = _torch()
         from concurrent.futures import ThreadPoolExecutor
-        with ThreadPoolExecutor(max_workers=8) as pool:  # NumPy releases the GIL in the sorts
-            ins = list(pool.map(lambda c: case_inputs(c, grid=self.grid), cases))
-        g = graph_to_device(concat_graphs([i[1] for i in ins]))
-        x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
-        with torch.no_grad():
-            y = self.forward(x, g).cpu().numpy().astype(np.float64) * self.target_scale
-        out, o = [], 0
-        for c, (_, gi, N) in zip(cases, ins):
-            yy = y[o:o + N] * c.u_in
-            o += gi["n"]
-            r = (1.0 + yy[:, 0]).astype(np.float32)
-            v = yy[:, 1:3].astype(np.float32)
-            out.append((r, project_mass_flux(c, r, v) if project else v))
+        out = []
+        with ThreadPoolExecutor(max_workers=8) as pool:
+            futs = [pool.submit(case_inputs, c, self.grid) for c in cases]
+            for c0 in range(0, len(cases), chunk):
+                ins = [f.result() for f in futs[c0:c0 + chunk]]
+                g = graph_to_device(concat_graphs([i[1] for i in ins]))
+                x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
+                with torch.no_grad():
+                    y = self.forward(x, g).cpu().numpy().astype(np.float64) * self.target_scale
+                o = 0
+                for c, (_, gi, N) in zip(cases[c0:c0 + chunk], ins):
+                    yy = y[o:o + N] * c.u_in
+                    o += gi["n"]
+                    r = (1.0 + yy[:, 0]).astype(np.float32)
+                    v = yy[:, 1:3].astype(np.float32)
+                    out.append((r, project_mass_flux(c, r, v) if project else v))
         return out
 
 
