@@ -5,7 +5,8 @@ Candidates combine
     also the wavefronts per tile: block / 64),
   * the kernel variant bits: 1 = no XCD-aware tile order, 2 = boundary
     nodes in their own kernel instead of the fused node phase, 4 = no LDS
-    staging of the tile's cells and halo,
+    staging of the tile's cells and halo, 8 = one step per launch instead
+    of two-step tiles,
   * the device cell order: 4 = Hilbert curve, 3 = Hilbert with boundary
     tiles first, 1 = reverse Cuthill-McKee.
 Every candidate must give the same bits: after ``check_steps`` steps its
@@ -27,7 +28,7 @@ from .environment import Environment
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "autotune.json")
 BLOCKS = (128, 256, 512)
-VARIANTS = (0, 1, 4)
+VARIANTS = (0, 8, 1, 4)
 REORDERS = (4, 3, 1)
 
 

@@ -347,6 +347,26 @@ struct StepArgs {
   Lattice<T> k;
 };
 
+// Two-step tiles (k_step2): ring lists, ring face records and the extended
+// node tables (see HostLayout in layout.h).
+template <typename T>
+struct PairArgs {
+  const int32_t *__restrict__ ring_off;    // [ntiles+1] into ring_cells
+  const int32_t *__restrict__ ring_cells;  // S1 \ O then S2, device ids
+  const int32_t *__restrict__ ring_n1;     // [ntiles] |S1 \ O|
+  const int32_t *__restrict__ rg_off;      // [ntiles] first ring face record of the tile
+  const void *rg_geo;                      // [3][nrg_cells] face records of S1 \ O cells
+  const int32_t *__restrict__ rgb_slot;    // [2][nrgb] extended-table slots
+  const T *__restrict__ rgb_ratio;         // [nrgb]
+  const int32_t *__restrict__ xnode_off, *__restrict__ xnodes, *__restrict__ xentry_pair_off,
+      *__restrict__ xpair_off, *__restrict__ xpair_cell;
+  const T *__restrict__ xpair_w;
+  const int32_t *__restrict__ pair_slot;   // stage slot of each own-table pair
+  int nrg_cells;                           // ring cells with face records (all tiles)
+  int nrgb;
+  int max_xnodes, max_xpairs, max_ring;
+};
+
 // The 9 pdfs of cell c from a q-major [9][Cp] buffer.
 template <typename T>
 __device__ __forceinline__ void load_cell(const Buf &f, uint32_t plane, int c, T (&out)[Q]) {
@@ -694,6 +714,227 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   if constexpr (RESID) block_reduce_store(res_d, res_n, a.partials + 2 * (size_t)tile);
 }
 
+// One upwind cell update with every neighbour in the LDS stage and the
+// ghost-face node values in the LDS node table (rows of length nrow): the
+// arithmetic of k_step<T, 0, ..> phase 2 with staging and inline nodes,
+// operation for operation.
+template <typename T>
+__device__ __forceinline__ void cell_update_staged(const Lattice<T> &k, const T (&f)[Q],
+                                                   const FaceRec<T> (&fr)[3], const T *shf,
+                                                   const T *tab, int nrow,
+                                                   const int32_t *__restrict__ gslot, int gstride,
+                                                   const T *__restrict__ gratio, T invtau, T dt,
+                                                   T (&fn)[Q]) {
+  T acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = T(0);
+#pragma unroll
+  for (int kf = 0; kf < 3; ++kf) {
+    const int32_t code = fr[kf].code;
+    const T nx = fr[kf].nx, ny = fr[kf].ny, sL = fr[kf].sL;
+    const bool slot1 = (code & kSlot1Bit) != 0;
+    T o[Q];
+    if (code >= 0) {
+      LdsCell<T>::get(shf, code & kIndexMask, o);
+    } else {
+      const int bf = code & kIndexMask;
+      const T ratio = gratio[bf];
+      const int s0 = gslot[bf], s1 = gslot[gstride + bf];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
+        o[q] = fbar + (fbar - f[q]) * ratio;
+      }
+    }
+    const T p5 = nx + ny, p6 = ny - nx;
+    const T phi[Q] = {T(0), nx, ny, -nx, -ny, p5, p6, -p5, -p6};
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+      const bool take_slot0 = phi[q] >= T(0);
+      const T up = (take_slot0 != slot1) ? f[q] : o[q];
+      acc[q] += (up * phi[q]) * sL;
+    }
+  }
+  T rho, ux, uy, feq[Q];
+  macro(f, rho, ux, uy);
+  equilibrium(k, rho, ux, uy, feq);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fn[q] = f[q] + dt * (invtau * (feq[q] - f[q]) - acc[q]);
+}
+
+// Node values of table entries [0, nent) (rows of length nrow) from their
+// (node, cell) pairs [pb, pb + np) (entry_pair_off absolute), the pair
+// cells read from global memory (FROM_LDS false: pair_cell device ids,
+// -2 - id for a weighted padding entry) or from the LDS stage (pair_slot):
+// k_step's phase 1 arithmetic. Ends with a barrier.
+template <typename T, bool FROM_LDS>
+__device__ __forceinline__ void node_table(const StepArgs<T> &a, T *tab, int nrow, T *con,
+                                           const int32_t *__restrict__ entries, int nent,
+                                           const int32_t *__restrict__ epo, int pb, int np,
+                                           const int32_t *__restrict__ pcell,
+                                           const int32_t *__restrict__ pslot,
+                                           const T *__restrict__ pw, const Buf &fin,
+                                           uint32_t plane, const T *shf) {
+  const int bs = blockDim.x;
+  for (int p = threadIdx.x; p < np; p += bs) {
+    const int cj = pcell[pb + p];
+    const T wj = pw[pb + p];
+    T fc[Q], r, ux, uy;
+    if constexpr (FROM_LDS) {
+      LdsCell<T>::get(shf, pslot[pb + p], fc);
+    } else {
+      load_cell(fin, plane, cj < 0 ? -2 - cj : cj, fc);
+    }
+    macro(fc, r, ux, uy);
+    T *o = con + kPairW * p;
+    o[0] = r * wj;
+    o[1] = ux * wj;
+    o[2] = uy * wj;
+    if (cj >= 0) {
+      T eq[Q];
+      equilibrium(a.k, r, ux, uy, eq);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) o[3 + q] = (fc[q] - eq[q]) * wj;
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) o[3 + q] = T(0) * wj;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nent; i += bs) {
+    const int e = entries[i];
+    T out[Q];
+    if (e >= 0) {
+      const int type = a.an_type[e];
+      const int p0 = epo[i] - pb, p1 = epo[i + 1] - pb;
+      T n0 = T(0), n1 = T(0), acc[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) acc[q] = T(0);
+      for (int p = p0; p < p1; ++p) {
+        const T *o = con + kPairW * p;
+        if (type == 1) {
+          n0 += o[0];
+        } else {
+          n0 += o[1];
+          n1 += o[2];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] += o[3 + q];
+      }
+      const T wsum = a.an_wsum[e];
+      T rho_n = a.an_rho[e], ux_n = a.an_ux[e], uy_n = a.an_uy[e];
+      if (type == 1) {
+        rho_n = n0 / wsum;
+      } else {
+        ux_n = n0 / wsum;
+        uy_n = n1 / wsum;
+      }
+      T eqn[Q];
+      equilibrium(a.k, rho_n, ux_n, uy_n, eqn);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) out[q] = eqn[q] + acc[q] / wsum;
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) out[q] = a.node_pdf[(size_t)q * a.N + (-1 - e)];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) tab[q * nrow + i] = out[q];
+  }
+  __syncthreads();
+}
+
+// Two steps of one tile (upwind, single handle, staged, inline nodes):
+// step n for S1 = the tile and its ring from the inputs of S1 and S2 (the
+// extended node table from global memory), the results kept in the LDS
+// stage, then the own node table from those results and step n + 1 for
+// the tile's cells, written to fdst. The intermediate state never goes to
+// HBM; every value equals what two launches of k_step compute.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sh = reinterpret_cast<T *>(smem_raw);
+  const int tile = tile_of_block(a);
+  if (tile < 0 || tile >= a.ntiles) return;  // uniform per block
+  const T invtau = a.k.invtau, dt = a.k.dt;
+  const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
+  const Buf fin(a.fsrc, Q * plane);
+  const int bs = blockDim.x;
+  const bool live = (int)threadIdx.x < a.tile_count[tile];
+  const int x0 = p.xnode_off[tile], nx = p.xnode_off[tile + 1] - x0;
+  const int t0 = a.tile_node_off[tile], ntn = a.tile_node_off[tile + 1] - t0;
+  T *con = sh + Q * p.max_xnodes;
+  const int pre = (Q * p.max_xnodes + kPairW * p.max_xpairs + 7) & ~7;
+  T *shf = static_cast<T *>(__builtin_assume_aligned(sh + pre, 16));
+  // step-n node values of every ghost face of S1, from global memory
+  if (nx > 0) {
+    const int pb = p.xpair_off[tile];
+    node_table<T, false>(a, sh, nx, con, p.xnodes + x0, nx, p.xentry_pair_off + x0, pb,
+                         p.xpair_off[tile + 1] - pb, p.xpair_cell, nullptr, p.xpair_w, fin, plane,
+                         shf);
+  }
+  // stage: own cells, then S1 \ O and S2
+  const int c = tile * bs + threadIdx.x;
+  T f[Q];
+  FaceRec<T> fr[3] = {};
+  if (live) {
+    load_cell(fin, plane, c, f);
+    const Buf geo(a.geo, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
+    const uint32_t gplane = (uint32_t)a.Cp * GeoLayout<T>::bytes;
+    const uint32_t gvo = (uint32_t)c * GeoLayout<T>::bytes;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) fr[kf] = GeoLayout<T>::load(geo, gvo, kf * gplane);
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = T(0);
+  }
+  LdsCell<T>::put(shf, threadIdx.x, f);
+  const int r0 = p.ring_off[tile], nr = p.ring_off[tile + 1] - r0, n1 = p.ring_n1[tile];
+  for (int i = threadIdx.x; i < nr; i += bs) {
+    T g[Q];
+    load_cell(fin, plane, p.ring_cells[r0 + i], g);
+    LdsCell<T>::put(shf, bs + i, g);
+  }
+  // the ring cell of this thread (n1 <= bs) and its face records
+  const bool ring_here = (int)threadIdx.x < n1;
+  FaceRec<T> rr[3] = {};
+  if (ring_here) {
+    const Buf rgeo(p.rg_geo, 3 * (uint32_t)p.nrg_cells * GeoLayout<T>::bytes);
+    const uint32_t rplane = (uint32_t)p.nrg_cells * GeoLayout<T>::bytes;
+    const uint32_t rvo = (uint32_t)(p.rg_off[tile] + (int)threadIdx.x) * GeoLayout<T>::bytes;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) rr[kf] = GeoLayout<T>::load(rgeo, rvo, kf * rplane);
+  }
+  __syncthreads();
+  // step n for the own cell and the ring cell of this thread
+  T fo[Q], fq[Q];
+  if (live)
+    cell_update_staged(a.k, f, fr, shf, sh, nx, a.bf_slot, a.nbf, a.bf_ratio, invtau, dt, fo);
+  if (ring_here) {
+    T g[Q];
+    LdsCell<T>::get(shf, bs + threadIdx.x, g);
+    cell_update_staged(a.k, g, rr, shf, sh, nx, p.rgb_slot, p.nrgb, p.rgb_ratio, invtau, dt, fq);
+  }
+  __syncthreads();  // every step-n input has been read
+  if (live) LdsCell<T>::put(shf, threadIdx.x, fo);
+  if (ring_here) LdsCell<T>::put(shf, bs + threadIdx.x, fq);
+  __syncthreads();
+  // step-(n+1) node values of the own ghost faces, from the stage
+  if (ntn > 0) {
+    const int pb = a.tile_pair_off[tile];
+    node_table<T, true>(a, sh, nx, con, a.tile_nodes + t0, ntn, a.entry_pair_off + t0, pb,
+                        a.tile_pair_off[tile + 1] - pb, a.pair_cell, p.pair_slot, a.pair_w, fin,
+                        plane, shf);
+  }
+  if (live) {
+    T fn[Q];
+    cell_update_staged(a.k, fo, fr, shf, sh, nx, a.bf_slot, a.nbf, a.bf_ratio, invtau, dt, fn);
+    const Buf fout(a.fdst, Q * plane);
+    const uint32_t vo = (uint32_t)c * sizeof(T);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fout.st<T>(fn[q], vo, q * plane);
+  }
+}
+
 // Active nodes from the pdf in fsrc: node pdf (type 1/2), node rho (type 1)
 // and node vel (type 2) written to the node arrays (reference node order).
 template <typename T>
@@ -1033,6 +1274,17 @@ struct Handle {
   void *partials = nullptr, *resid = nullptr;
   void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
   void *tile_count = nullptr;
+  // two-step tiles (k_step2): used unless variant bit 8 is set, for single
+  // upwind handles whose rings and LDS fit
+  bool pairs = false;
+  size_t lds2 = 0;
+  int nrg_cells = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0, max_ring = 0;
+  void *ring_off = nullptr, *ring_cells = nullptr, *ring_n1 = nullptr, *rg_off = nullptr,
+       *rg_geo = nullptr, *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr,
+       *xnodes = nullptr, *xentry_pair_off = nullptr, *xpair_off = nullptr,
+       *xpair_cell = nullptr, *xpair_w = nullptr, *pair_slot = nullptr;
+  hipGraphExec_t gexec2[2] = {nullptr, nullptr};
+  hipGraph_t graph2[2] = {nullptr, nullptr};
   // all-node interpolation: node -> (slot, d) lists [N][K], cell -> vertex
   // CSR in slot order, and lazily allocated work arrays
   void *all_nc = nullptr, *all_nd = nullptr, *cn_off = nullptr, *cn_node = nullptr, *cn_d = nullptr;
@@ -1067,6 +1319,8 @@ struct Handle {
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
+      if (gexec2[i]) (void)hipGraphExecDestroy(gexec2[i]);
+      if (graph2[i]) (void)hipGraphDestroy(graph2[i]);
     }
     if (bexec) (void)hipGraphExecDestroy(bexec);
     if (sexec) (void)hipGraphExecDestroy(sexec);
@@ -1200,6 +1454,41 @@ static int launch_step_any(Handle &h, int src, bool resid) {
                                    : launch_step<float>(h, src, resid);
 }
 
+template <typename T>
+static int launch_pair_typed(Handle &h, int src) {
+  StepArgs<T> a = make_args<T>(h, src);
+  PairArgs<T> p{};
+  p.ring_off = static_cast<const int32_t *>(h.ring_off);
+  p.ring_cells = static_cast<const int32_t *>(h.ring_cells);
+  p.ring_n1 = static_cast<const int32_t *>(h.ring_n1);
+  p.rg_off = static_cast<const int32_t *>(h.rg_off);
+  p.rg_geo = h.rg_geo;
+  p.rgb_slot = static_cast<const int32_t *>(h.rgb_slot);
+  p.rgb_ratio = static_cast<const T *>(h.rgb_ratio);
+  p.xnode_off = static_cast<const int32_t *>(h.xnode_off);
+  p.xnodes = static_cast<const int32_t *>(h.xnodes);
+  p.xentry_pair_off = static_cast<const int32_t *>(h.xentry_pair_off);
+  p.xpair_off = static_cast<const int32_t *>(h.xpair_off);
+  p.xpair_cell = static_cast<const int32_t *>(h.xpair_cell);
+  p.xpair_w = static_cast<const T *>(h.xpair_w);
+  p.pair_slot = static_cast<const int32_t *>(h.pair_slot);
+  p.nrg_cells = h.nrg_cells;
+  p.nrgb = h.nrgb;
+  p.max_xnodes = h.max_xnodes;
+  p.max_xpairs = h.max_xpairs;
+  p.max_ring = h.max_ring;
+  hipLaunchKernelGGL((k_step2<T>), dim3(grid_for(h, a.xcd_remap)), dim3(h.block), h.lds2, h.stream,
+                     a, p);
+  HIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+// Two steps reading f[src] and writing f[src ^ 1].
+static int launch_pair(Handle &h, int src) {
+  return h.precision == FVDBM_FP64 ? launch_pair_typed<double>(h, src)
+                                   : launch_pair_typed<float>(h, src);
+}
+
 static int launch_batch_advance(Handle &h, int n) {
   hipLaunchKernelGGL(k_batch_advance, dim3((h.ncases + 255) / 256), dim3(256), 0, h.stream,
                      h.ncases, static_cast<const int32_t *>(h.case_active),
@@ -1289,6 +1578,28 @@ static int ensure_graph(Handle &h, int parity) {
   return FVDBM_OK;
 }
 
+// Two-step tiles: graph_steps / 2 pair launches starting from `parity`.
+static int ensure_graph2(Handle &h, int parity) {
+  if (h.gexec2[parity]) return FVDBM_OK;
+  HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  int src = parity;
+  int rc = FVDBM_OK;
+  for (int s = 0; s < h.graph_steps / 2 && rc == FVDBM_OK; ++s) {
+    rc = launch_pair(h, src);
+    src ^= 1;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h.graph2[parity] = g;
+  HIP_TRY(hipGraphInstantiate(&h.gexec2[parity], g, nullptr, nullptr, 0));
+  return FVDBM_OK;
+}
+
 // Batched: a graph of graph_steps steps and the advance, replayed.
 static int ensure_batch_step_graph(Handle &h) {
   if (h.sexec) return FVDBM_OK;
@@ -1331,6 +1642,28 @@ static int run_batch_steps(Handle &h, int64_t n, bool resid_last) {
 static int run_steps(Handle &h, int64_t n, bool resid_last) {
   if (n <= 0) return FVDBM_OK;
   if (h.batch) return run_batch_steps(h, n, resid_last);
+  if (h.pairs && n >= 3) {
+    // pairs of steps for all but the last one or two, so the last step is
+    // a single launch and f[cur ^ 1] keeps the pdf at its start
+    int64_t rest = n - 1;
+    if (h.graph_steps >= 2) {
+      while (rest >= h.graph_steps) {
+        int rc = ensure_graph2(h, h.cur);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
+        h.cur ^= (h.graph_steps / 2) & 1;
+        rest -= h.graph_steps;
+        h.steps += h.graph_steps;
+      }
+    }
+    for (; rest >= 2; rest -= 2) {
+      int rc = launch_pair(h, h.cur);
+      if (rc) return rc;
+      h.cur ^= 1;
+      h.steps += 2;
+    }
+    n = rest + 1;  // the remaining single steps, the last included
+  }
   int64_t plain = resid_last ? n - 1 : n;
   if (h.graph_steps > 0 && !h.batch) {
     while (plain >= h.graph_steps) {
@@ -1529,6 +1862,53 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
     if (!rc) rc = h.upload(&h.cn_node, cn_node);
     if (!rc) rc = h.upload(&h.cn_d, cn_d);
   }
+  if (!rc && !(h.variant & 8) && !cell_case && h.scheme == FVDBM_UPWIND && !(h.variant & 6) &&
+      L.pair_ok) {
+    // two-step tiles, when their LDS fits a block
+    const size_t pre2 = ((size_t)Q * L.max_xnodes + kPairW * (size_t)L.max_xpairs + 7) & ~(size_t)7;
+    const size_t lds2 = (pre2 + (size_t)9 * (h.block + L.max_ring)) * es;
+    if (lds2 <= 65536) {
+      const int nrgc = (int)(L.rg_code.size() / 3);
+      const size_t rec = GeoLayout<T>::bytes;
+      std::vector<unsigned char> g(3 * (size_t)std::max(nrgc, 1) * rec, 0);
+      for (int r = 0; r < nrgc; ++r)
+        for (int k = 0; k < 3; ++k) {
+          unsigned char *d = g.data() + ((size_t)k * nrgc + r) * rec;
+          const size_t si = 3 * (size_t)r + k;
+          std::memcpy(d, &L.rg_code[si], 4);
+          const size_t o = sizeof(T) == 4 ? 4 : 8;
+          std::memcpy(d + o, &L.rg_nx[si], sizeof(T));
+          std::memcpy(d + o + sizeof(T), &L.rg_ny[si], sizeof(T));
+          std::memcpy(d + o + 2 * sizeof(T), &L.rg_sL[si], sizeof(T));
+        }
+      std::vector<int32_t> rg_off(L.ntiles, 0);
+      for (int t = 1; t < L.ntiles; ++t) rg_off[t] = rg_off[t - 1] + L.ring_n1[t - 1];
+      rc = h.upload(&h.rg_geo, g);
+      if (!rc) rc = h.upload(&h.rg_off, rg_off);
+      if (!rc) rc = h.upload(&h.ring_off, L.ring_off);
+      if (!rc) rc = h.upload(&h.ring_cells, L.ring_cells);
+      if (!rc) rc = h.upload(&h.ring_n1, L.ring_n1);
+      if (!rc) rc = h.upload(&h.rgb_slot, L.rgb_slot);
+      if (!rc) rc = h.upload(&h.rgb_ratio, L.rgb_ratio);
+      if (!rc) rc = h.upload(&h.xnode_off, L.xnode_off);
+      if (!rc) rc = h.upload(&h.xnodes, L.xnodes);
+      if (!rc) rc = h.upload(&h.xentry_pair_off, L.xentry_pair_off);
+      if (!rc) rc = h.upload(&h.xpair_off, L.xpair_off);
+      if (!rc) rc = h.upload(&h.xpair_cell, L.xpair_cell);
+      if (!rc) rc = h.upload(&h.xpair_w, L.xpair_w);
+      if (!rc) rc = h.upload(&h.pair_slot, L.pair_slot);
+      if (!rc) {
+        h.pairs = true;
+        h.lds2 = lds2;
+        h.nrg_cells = nrgc;
+        h.nrgb = L.nrg;
+        h.max_xnodes = L.max_xnodes;
+        h.max_xpairs = L.max_xpairs;
+        h.max_ring = L.max_ring;
+      }
+    }
+  }
+  if (!rc && !h.pairs) h.variant |= 8;  // one step per launch (two-step tiles off or unavailable)
   if (rc) return rc;
   // node rho/vel hold the boundary-condition values (reference node order)
   std::vector<T> rho(h.N), vel(2 * (size_t)h.N);
@@ -1946,7 +2326,9 @@ int fvdbm_device_count(int32_t *count) {
 
 int fvdbm_variant_count(int32_t *count) {
   if (!count) return fail(FVDBM_ERR_INVALID, "null count");
-  *count = 8;  // bit 0: no XCD remap; bit 1: separate node kernel; bit 2: no LDS staging
+  // bit 0: no XCD remap; bit 1: separate node kernel; bit 2: no LDS staging;
+  // bit 3: one step per launch (no two-step tiles)
+  *count = 16;
   return FVDBM_OK;
 }
 
@@ -1963,7 +2345,7 @@ static int create_impl(const fvdbm_mesh_desc *mesh, const fvdbm_params *p,
   int block = p->block ? p->block : 256;
   if (block < 64 || block > 1024 || block % 64)
     return fail(FVDBM_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
-  if (p->variant < 0 || p->variant > 7) return fail(FVDBM_ERR_INVALID, "variant out of range");
+  if (p->variant < 0 || p->variant > 15) return fail(FVDBM_ERR_INVALID, "variant out of range");
   int gs = p->graph_steps == 0 ? 32 : p->graph_steps;
   if (gs > 0 && (gs % 2)) ++gs;  // even, so a replay preserves buffer parity
   std::unique_ptr<Handle> h(new Handle());
@@ -2190,7 +2572,9 @@ int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms) {
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   // build graphs outside the timed region
   if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
-    int rc = h->batch ? fvdbm::ensure_batch_step_graph(*h) : fvdbm::ensure_graph(*h, h->cur);
+    int rc = h->batch  ? fvdbm::ensure_batch_step_graph(*h)
+             : h->pairs ? fvdbm::ensure_graph2(*h, h->cur)
+                        : fvdbm::ensure_graph(*h, h->cur);
     if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));

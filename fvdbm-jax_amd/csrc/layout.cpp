@@ -130,6 +130,176 @@ std::vector<int32_t> reverse_cuthill_mckee(int n, const std::vector<int32_t> &xa
   return order;
 }
 
+// Two-step tiles (see HostLayout): ring lists, ring face records, extended
+// node tables and stage slots of the own-table pairs. Leaves pair_ok false
+// when some tile's ring does not fit the one-ring-cell-per-thread pass.
+template <typename T>
+static void build_pairs(const fvdbm_mesh_desc &m, HostLayout<T> &L, int pad,
+                        const std::vector<int32_t> &other, const std::vector<int32_t> &slot) {
+  const T *fd = static_cast<const T *>(m.face_dists);
+  const T *fn = static_cast<const T *>(m.face_n);
+  const T *fL = static_cast<const T *>(m.face_L);
+  const int K = L.K;
+  const size_t Cp = (size_t)L.Cp;
+  L.pair_ok = true;
+  L.max_ring = 0;
+  L.ring_off.assign(L.ntiles + 1, 0);
+  L.ring_cells.clear();
+  L.ring_n1.assign(L.ntiles, 0);
+  L.rg_code.clear();
+  L.rg_nx.clear();
+  L.rg_ny.clear();
+  L.rg_sL.clear();
+  L.rgb_slot.clear();
+  L.rgb_ratio.clear();
+  std::vector<int32_t> rgb0, rgb1;
+  L.xnode_off.assign(L.ntiles + 1, 0);
+  L.xnodes.clear();
+  L.xentry_pair_off.clear();
+  L.xpair_off.assign(L.ntiles + 1, 0);
+  L.xpair_cell.clear();
+  L.xpair_w.clear();
+  L.pair_slot.assign(L.pair_cell.size(), -1);
+  L.max_xnodes = 0;
+  L.max_xpairs = 0;
+  std::vector<int32_t> node_an(L.N, -1);
+  for (int a = 0; a < L.nan; ++a) node_an[L.an_gid[a]] = a;
+  std::vector<int32_t> stage_of(Cp, -1);  // device cell -> stage slot in the current tile
+  std::vector<int32_t> xslot_of(L.N, -1);  // node -> slot in the current extended table
+  std::vector<int32_t> ring, s2, touched_nodes;
+  auto pair_cell_id = [&](int32_t cj) { return cj >= 0 ? cj : -2 - cj; };
+  for (int t = 0; t < L.ntiles; ++t) {
+    const int lo = t * pad, hi = lo + L.tile_count[t];
+    for (int i = lo; i < hi; ++i) stage_of[i] = i - lo;
+    // S1 \ O: the halo (ascending, as staged by the one-step kernel), then
+    // the other cells the own node table's pairs read (ascending)
+    ring.assign(L.halo_cells.begin() + L.halo_off[t], L.halo_cells.begin() + L.halo_off[t + 1]);
+    for (size_t r = 0; r < ring.size(); ++r) stage_of[ring[r]] = pad + (int)r;
+    std::vector<int32_t> f1;
+    for (int p = L.tile_pair_off[t]; p < L.tile_pair_off[t + 1]; ++p) {
+      const int c = pair_cell_id(L.pair_cell[p]);
+      if (stage_of[c] < 0) f1.push_back(c);
+    }
+    std::sort(f1.begin(), f1.end());
+    f1.erase(std::unique(f1.begin(), f1.end()), f1.end());
+    for (int c : f1) {
+      stage_of[c] = pad + (int)ring.size();
+      ring.push_back(c);
+    }
+    const int n1 = (int)ring.size();
+    // S2: face neighbours of S1 \ O outside S1
+    s2.clear();
+    for (int r = 0; r < n1; ++r) {
+      const int c = L.new_to_old[ring[r]];
+      for (int k = 0; k < 3; ++k) {
+        const int o = other[3 * c + k];
+        if (o < 0) continue;
+        const int j = L.old_to_new[o];
+        if (stage_of[j] < 0) s2.push_back(j);
+      }
+    }
+    std::sort(s2.begin(), s2.end());
+    s2.erase(std::unique(s2.begin(), s2.end()), s2.end());
+    for (size_t r = 0; r < s2.size(); ++r) stage_of[s2[r]] = pad + n1 + (int)r;
+    if (n1 > pad) L.pair_ok = false;
+    L.ring_n1[t] = n1;
+    L.ring_off[t] = (int32_t)L.ring_cells.size();
+    L.ring_cells.insert(L.ring_cells.end(), ring.begin(), ring.end());
+    L.ring_cells.insert(L.ring_cells.end(), s2.begin(), s2.end());
+    L.max_ring = std::max(L.max_ring, n1 + (int)s2.size());
+    // extended node table: the own table, then the ring cells' ghost nodes
+    L.xnode_off[t] = (int32_t)L.xnodes.size();
+    touched_nodes.clear();
+    for (int e = L.tile_node_off[t]; e < L.tile_node_off[t + 1]; ++e) {
+      const int v = L.tile_nodes[e];
+      const int node = v >= 0 ? L.an_gid[v] : -1 - v;
+      xslot_of[node] = e - L.tile_node_off[t];
+      touched_nodes.push_back(node);
+      L.xnodes.push_back(v);
+    }
+    auto xslot = [&](int node) {
+      if (xslot_of[node] < 0) {
+        xslot_of[node] = (int32_t)(L.xnodes.size() - L.xnode_off[t]);
+        L.xnodes.push_back(node_an[node] >= 0 ? node_an[node] : -1 - node);
+        touched_nodes.push_back(node);
+      }
+      return xslot_of[node];
+    };
+    // ring face records
+    for (int r = 0; r < n1; ++r) {
+      const int c = L.new_to_old[ring[r]];
+      for (int k = 0; k < 3; ++k) {
+        const int fid = m.cell_faces[3 * c + k];
+        const int sl = slot[3 * c + k], o = other[3 * c + k];
+        int32_t code;
+        if (o >= 0) {
+          code = stage_of[L.old_to_new[o]];
+        } else {
+          const T d0 = fd[2 * (size_t)fid], d1 = fd[2 * (size_t)fid + 1];
+          code = kGhostBit | (int32_t)L.rgb_ratio.size();
+          L.rgb_ratio.push_back(sl == 0 ? d1 / d0 : d0 / d1);
+          rgb0.push_back(xslot(m.face_nodes[2 * (size_t)fid]));
+          rgb1.push_back(xslot(m.face_nodes[2 * (size_t)fid + 1]));
+        }
+        if (sl == 1) code |= kSlot1Bit;
+        L.rg_code.push_back(code);
+        L.rg_nx.push_back(fn[2 * (size_t)fid]);
+        L.rg_ny.push_back(fn[2 * (size_t)fid + 1]);
+        L.rg_sL.push_back(T(m.cell_signs[3 * c + k]) * fL[fid]);
+      }
+    }
+    // pairs of the extended table (own entries: the own pairs)
+    L.xpair_off[t] = (int32_t)L.xpair_cell.size();
+    for (size_t e = L.xnode_off[t]; e < L.xnodes.size(); ++e) {
+      L.xentry_pair_off.push_back((int32_t)L.xpair_cell.size());
+      const int an = L.xnodes[e];
+      if (an < 0) continue;
+      for (int k = 0; k < K; ++k) {
+        const int32_t cj = L.an_cells[(size_t)an * K + k];
+        const T w = L.an_w[(size_t)an * K + k];
+        if (cj < 0 && w == T(0)) continue;
+        L.xpair_cell.push_back(cj);
+        L.xpair_w.push_back(w);
+      }
+    }
+    L.max_xnodes = std::max(L.max_xnodes, (int)L.xnodes.size() - L.xnode_off[t]);
+    L.max_xpairs = std::max(L.max_xpairs, (int)L.xpair_cell.size() - L.xpair_off[t]);
+    // stage slots of the own pairs
+    for (int p = L.tile_pair_off[t]; p < L.tile_pair_off[t + 1]; ++p)
+      L.pair_slot[p] = stage_of[pair_cell_id(L.pair_cell[p])];
+    // reset the per-tile maps
+    for (int i = lo; i < hi; ++i) stage_of[i] = -1;
+    for (int c : ring) stage_of[c] = -1;
+    for (int c : s2) stage_of[c] = -1;
+    for (int nd_ : touched_nodes) xslot_of[nd_] = -1;
+  }
+  L.ring_off[L.ntiles] = (int32_t)L.ring_cells.size();
+  L.xnode_off[L.ntiles] = (int32_t)L.xnodes.size();
+  L.xpair_off[L.ntiles] = (int32_t)L.xpair_cell.size();
+  L.xentry_pair_off.push_back((int32_t)L.xpair_cell.size());
+  L.nrg = (int)L.rgb_ratio.size();
+  L.rgb_slot.resize(2 * (size_t)L.nrg);
+  std::copy(rgb0.begin(), rgb0.end(), L.rgb_slot.begin());
+  std::copy(rgb1.begin(), rgb1.end(), L.rgb_slot.begin() + L.nrg);
+  // defensive range checks of everything k_step2 indexes with these tables
+  for (int t = 0, rc = 0; t < L.ntiles && L.pair_ok; ++t) {
+    const int nr = L.ring_off[t + 1] - L.ring_off[t], n1 = L.ring_n1[t];
+    const int nx = L.xnode_off[t + 1] - L.xnode_off[t];
+    for (int r = 0; r < n1; ++r, ++rc)
+      for (int k = 0; k < 3; ++k) {
+        const int32_t code = L.rg_code[3 * (size_t)rc + k];
+        if (code >= 0) {
+          if ((code & kIndexMask) >= pad + nr) L.pair_ok = false;
+        } else {
+          const int b = code & kIndexMask;
+          if (b >= L.nrg || L.rgb_slot[b] >= nx || L.rgb_slot[L.nrg + b] >= nx) L.pair_ok = false;
+        }
+      }
+    for (int q = L.tile_pair_off[t]; q < L.tile_pair_off[t + 1]; ++q)
+      if (L.pair_slot[q] < 0 || L.pair_slot[q] >= pad + n1) L.pair_ok = false;
+  }
+}
+
 template <typename T>
 bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
                   HostLayout<T> &L, std::string &err, const int32_t *cell_case, int ncases) {
@@ -472,6 +642,8 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
   std::copy(bs0.begin(), bs0.end(), L.bf_slot.begin());
   std::copy(bs1.begin(), bs1.end(), L.bf_slot.begin() + L.nbf);
   L.bandwidth = bw;
+
+  if (!cell_case) build_pairs(m, L, pad, other, slot);
 
   // --- faces in reference order (for the flux observable)
   L.fc0.resize(F);

@@ -75,6 +75,34 @@ struct HostLayout {
   std::vector<T> an_wsum;         // [nan]     0-initialised ordered sum of an_w
   std::vector<T> an_rho, an_ux, an_uy;
 
+  // Two-step tiles (one launch advances a tile by two steps). With O the
+  // tile's cells: S1 = O, its face neighbours (the halo, same order) and
+  // every cell a node of an O ghost face reads; S2 = the face neighbours of
+  // S1 outside it. Step n is computed for S1 from the inputs of S1 and S2,
+  // step n + 1 for O from those results. Per tile, ring = S1 \ O then S2
+  // (device ids), staged after the tile's own slots; n1 = |S1 \ O|.
+  bool pair_ok = false;       // every tile fits (n1 <= tile, stage within bounds)
+  int max_ring = 0;           // max |S1 \ O| + |S2|
+  std::vector<int32_t> ring_off, ring_cells, ring_n1;
+  // face records of the S1 \ O cells ([3] per ring cell, in ring order):
+  // code = stage slot of the neighbour | slot-1 bit, or ghost bit | index
+  // into the ring-ghost records; normal and sign * length as for own cells
+  std::vector<int32_t> rg_code;
+  std::vector<T> rg_nx, rg_ny, rg_sL;
+  // ring-ghost records: [2][nrg] endpoint slots in the tile's extended node
+  // table, [nrg] ghost/known distance ratio
+  int nrg = 0;
+  std::vector<int32_t> rgb_slot;
+  std::vector<T> rgb_ratio;
+  // extended node table of each tile: its own table (same slots) followed
+  // by the nodes of the ring cells' ghost faces; pairs as for the own table
+  int max_xnodes = 0, max_xpairs = 0;
+  std::vector<int32_t> xnode_off, xnodes, xentry_pair_off, xpair_off, xpair_cell;
+  std::vector<T> xpair_w;
+  // stage slot of each (node, cell) pair of the own tables (parallel to
+  // pair_cell), for the node values at the start of the second step
+  std::vector<int32_t> pair_slot;
+
   // faces in reference order (flux observable)
   std::vector<int32_t> fc0, fc1, fn0, fn1;
   std::vector<T> fd0, fd1, fnx, fny, fL;

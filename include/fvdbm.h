@@ -91,7 +91,10 @@ typedef struct fvdbm_params {
                          centroids with ghost-face cells first, 4 Hilbert curve
                          (3 and 4 need cell_xy) */
   int32_t variant;    /* step kernel variant bits, 0 = default: 1 no XCD-aware tile order,
-                         2 boundary nodes in their own kernel, 4 no LDS staging */
+                         2 boundary nodes in their own kernel, 4 no LDS staging,
+                         8 one step per launch (by default single upwind handles
+                         advance two steps per launch where the tiles fit; the
+                         handle's info reports bit 8 when they do not) */
   int32_t block;      /* threads per block (64..1024, multiple of 64), 0 = default */
   int32_t graph_steps;/* steps per captured hipGraph, 0 = default, <0 = no graphs */
 } fvdbm_params;

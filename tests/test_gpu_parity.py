@@ -121,10 +121,12 @@ def test_channel_matches_oracle(gpu, scheme):
     dict(reorder=0), dict(reorder=1), dict(reorder=2), dict(reorder=3), dict(variant=1), dict(variant=2),
     dict(variant=3), dict(variant=4), dict(variant=7), dict(reorder=3, block=1024),
     dict(block=64), dict(block=512), dict(block=1024), dict(graph_steps=-1), dict(graph_steps=6),
+    dict(variant=8), dict(variant=9), dict(block=128), dict(graph_steps=10),
 ])
 def test_layout_and_variants_bitwise_equal(gpu, opts):
-    """Cell order (n1), kernel variant / block size (autotuner candidates) and
-    graph capture change nothing in the results."""
+    """Cell order (n1), kernel variant / block size (autotuner candidates),
+    one or two steps per launch and graph capture change nothing in the
+    results."""
     m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=1)
     base = _gpu_env(cells, faces, nodes)
     base.step(77)
