@@ -17,11 +17,12 @@ launch's duration, i.e. 108 B/LUP * cells / (elapsed/K).
 cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
 of the same mesh, on this host.
 
-batched: ``--batch-cases`` porous cases of the batched workload (5-40
-discs, 20k-200k cells, seeds 0..) in one handle: the autotuner picks tile
-size and kernel variant (every candidate checked bitwise), then MLUPS of the
-batched step over all their cells, then a masked run to a 1e-6 residual
-with a step cap (many of these cases have no steady state).
+batched: ``--batch-cases`` (default 256) porous cases of the batched
+workload (5-40 discs, 20k-200k cells, seeds 0..) in one handle: the
+autotuner picks tile size and kernel variant on the first 32 cases (every
+candidate checked bitwise), then MLUPS of the batched step over all their
+cells, then a masked run to a 1e-6 residual with a step cap (many of these
+cases have no steady state).
 
 steady_state (the second half of the metric): ``--steady-cases`` unseen
 porous-channel cases of the study distribution (seeds from 1,000,000; the
@@ -119,28 +120,33 @@ WEIGHTS = os.path.join(ROOT, "fvdbm-jax_amd", "data", "surrogate_porous.npz")
 
 
 def _build_workload(seed):
+    """(cells, faces, nodes) of batched-workload case ``seed`` (the mesher
+    and its geometry stay in the worker)."""
     fv = importlib.import_module("fvdbm-jax_amd")
-    return fv.porous.make_case(seed, dist=fv.porous.WORKLOAD)
+    return fv.porous.make_case(seed, dist=fv.porous.WORKLOAD).env_tuple()
 
 
-def batched(cases, steps=1000, masked_steps=20_000, check_every=500):
+def batched(triples, steps=1000, masked_steps=20_000, check_every=500, tune_cases=32):
     """Throughput of one batched launch over uneven porous cases (the
     batched workload's distribution), then a masked run: every case steps
     until its residual is below 1e-6, it diverges or grows 10x past its best,
-    or ``masked_steps`` have run."""
+    or ``masked_steps`` have run. The launch options are tuned on the first
+    ``tune_cases`` cases (every candidate checked bitwise)."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     _lib = importlib.import_module("fvdbm-jax_amd._lib")
     import ctypes as ct
     at = importlib.import_module("fvdbm-jax_amd.autotune")
-    triples = [c.env_tuple() for c in cases]
-    tuned = at.tune_batch(triples, cands=at.candidates(blocks=(128, 256, 512), variants=(0, 2, 4),
-                                                       reorders=(4,)),
+    tuned = at.tune_batch(triples[:tune_cases],
+                          cands=at.candidates(blocks=(128, 256, 512), variants=(0, 8, 2, 4),
+                                              reorders=(4,)),
                           steps=200, cache=None, log=log)
     best = tuned["best"]
     default = [r for r in tuned["results"] if (r["block"], r["variant"]) == (256, 0)]
+    t0 = time.perf_counter()
     b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
-    cells = int(sum(c.n_cells for c in cases))
+    log(f"batched handle: {len(triples)} cases uploaded in {time.perf_counter() - t0:.1f}s")
+    cells = int(sum(c.face_indices.shape[0] for c, _, _ in triples))
     info = _lib.Info()
     _lib.check(_lib.lib().fvdbm_info_get(b._handle(), ct.byref(info)), "fvdbm_info_get")
     b.step(100)
@@ -157,8 +163,9 @@ def batched(cases, steps=1000, masked_steps=20_000, check_every=500):
     counts = {ws_name: int((st["status"] == code).sum()) for code, ws_name in
               ((ws.RUNNING, "running"), (ws.CONVERGED, "converged"), (ws.DIVERGED, "diverged"),
                (ws.GROWING, "growing"))}
-    return dict(cases=len(cases), cells=cells, padded_cells=int(info.n_cells_padded), steps=steps,
-                tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"]),
+    return dict(cases=len(triples), cells=cells, padded_cells=int(info.n_cells_padded), steps=steps,
+                tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"],
+                           on_cases=min(tune_cases, len(triples))),
                 default_us_per_step=default[0]["us_per_step"] if default else None,
                 us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
                 hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
@@ -238,7 +245,7 @@ def main():
                     help="unseen porous cases for the cold/warm steady-state study (0: skip)")
     ap.add_argument("--steady-seed0", type=int, default=1_000_000)
     ap.add_argument("--weights", default=WEIGHTS)
-    ap.add_argument("--batch-cases", type=int, default=32,
+    ap.add_argument("--batch-cases", type=int, default=256,
                     help="porous cases of the batched workload for the batched throughput line (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:

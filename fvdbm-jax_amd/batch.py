@@ -100,6 +100,13 @@ class BatchSolver(_lib.FieldsMixin):
     def _counts(self):
         return int(self.cell_off[-1]), int(self.node_off[-1])
 
+    def info(self):
+        """Layout and launch facts of the handle (``variant`` has bit 8 set
+        when the steps run one per launch)."""
+        inf = _lib.Info()
+        _lib.check(_lib.lib().fvdbm_info_get(self._handle(), ct.byref(inf)), "fvdbm_info_get")
+        return {k: getattr(inf, k) for k, _ in inf._fields_}
+
     def case_nodes(self, k, arr):
         """Rows of node array ``arr`` that belong to case k."""
         return arr[self.node_off[k]:self.node_off[k + 1]]

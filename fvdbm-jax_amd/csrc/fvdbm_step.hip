@@ -340,7 +340,7 @@ struct StepArgs {
   const T *f0;  // the two pdf buffers (batched solves pick one per case)
   T *f1;
   int batch;
-  int bstep;                              // step index within the current batch interval
+  int bstep;                              // launch index within the current batch interval
   const int32_t *__restrict__ an_case;    // [nan] case of each active node (batched)
   int C, Cp, N, K, nbf, nan;
   int ntiles, xcd_remap, inline_nodes, stage, max_tile_nodes, max_tile_pairs, max_halo;
@@ -496,8 +496,9 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   if (a.batch) {  // the tile's case: skip it when frozen, else its parity and parameters
     const int cs = a.tile_case[tile];
     if (!a.case_active[cs]) return;  // uniform per block
-    // case_cur holds the parity at the start of the interval; the step
-    // index within it gives the current one (advanced after the interval)
+    // case_cur holds the parity at the start of the interval; the launch
+    // index within it (each launch swaps the buffers once) gives the
+    // current one (advanced after the interval)
     const int cur = (a.case_cur[cs] + a.bstep) & 1;
     fsrc = cur ? a.f1 : a.f0;
     fdst = cur ? const_cast<T *>(a.f0) : a.f1;
@@ -848,16 +849,29 @@ __device__ __forceinline__ void node_table(const StepArgs<T> &a, T *tab, int nro
 // extended node table from global memory), the results kept in the LDS
 // stage, then the own node table from those results and step n + 1 for
 // the tile's cells, written to fdst. The intermediate state never goes to
-// HBM; every value equals what two launches of k_step compute.
+// HBM; every value equals what two launches of k_step compute. Batched
+// handles: a.bstep is the launch index within the interval (as in k_step),
+// and a frozen case's tiles return at once.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sh = reinterpret_cast<T *>(smem_raw);
   const int tile = tile_of_block(a);
   if (tile < 0 || tile >= a.ntiles) return;  // uniform per block
-  const T invtau = a.k.invtau, dt = a.k.dt;
+  const T *fsrc = a.fsrc;
+  T *fdst = a.fdst;
+  T invtau = a.k.invtau, dt = a.k.dt;
+  if (a.batch) {  // ring cells and node-fan cells belong to the tile's case
+    const int cs = a.tile_case[tile];
+    if (!a.case_active[cs]) return;  // uniform per block
+    const int cur = (a.case_cur[cs] + a.bstep) & 1;
+    fsrc = cur ? a.f1 : a.f0;
+    fdst = cur ? const_cast<T *>(a.f0) : a.f1;
+    invtau = a.case_invtau[cs];
+    dt = a.case_dt[cs];
+  }
   const uint32_t plane = (uint32_t)a.Cp * sizeof(T);
-  const Buf fin(a.fsrc, Q * plane);
+  const Buf fin(fsrc, Q * plane);
   const int bs = blockDim.x;
   const bool live = (int)threadIdx.x < a.tile_count[tile];
   const int x0 = p.xnode_off[tile], nx = p.xnode_off[tile + 1] - x0;
@@ -928,7 +942,7 @@ __global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
   if (live) {
     T fn[Q];
     cell_update_staged(a.k, fo, fr, shf, sh, nx, a.bf_slot, a.nbf, a.bf_ratio, invtau, dt, fn);
-    const Buf fout(a.fdst, Q * plane);
+    const Buf fout(fdst, Q * plane);
     const uint32_t vo = (uint32_t)c * sizeof(T);
 #pragma unroll
     for (int q = 0; q < Q; ++q) fout.st<T>(fn[q], vo, q * plane);
@@ -1072,13 +1086,15 @@ __global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ 
     dst[i] = src[i];
 }
 
-// Batched solves, after an interval of n steps (launched with step index
-// 0..n-1): active cases advance their buffer parity and step count by n.
+// Batched solves, after an interval of n steps taken by `swaps` launches
+// (launch index 0..swaps-1; a two-step launch swaps the buffers once):
+// active cases advance their step count by n and their parity by swaps.
 __global__ void k_batch_advance(int ncases, const int32_t *__restrict__ active,
-                                int32_t *__restrict__ cur, int64_t *__restrict__ steps, int n) {
+                                int32_t *__restrict__ cur, int64_t *__restrict__ steps, int n,
+                                int swaps) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ncases || !active[k]) return;
-  cur[k] ^= (n & 1);
+  cur[k] ^= (swaps & 1);
   steps[k] += n;
 }
 
@@ -1093,13 +1109,13 @@ __global__ void k_batch_check(int ncases, const int32_t *__restrict__ case_tile_
                               int32_t *__restrict__ active, int32_t *__restrict__ status,
                               double *__restrict__ resid, double *__restrict__ best,
                               int32_t *__restrict__ n_active, int32_t *__restrict__ cur,
-                              int64_t *__restrict__ steps, int every) {
+                              int64_t *__restrict__ steps, int every, int swaps) {
   __shared__ int cnt[256];
   int mine = 0;
   for (int k = threadIdx.x; k < ncases; k += blockDim.x) {
     if (active[k]) {
       // the interval's steps: parity and count, as k_batch_advance
-      cur[k] ^= (every & 1);
+      cur[k] ^= (swaps & 1);
       steps[k] += every;
       double d = 0.0, n = 0.0;
       for (int t = case_tile_off[k]; t < case_tile_off[k + 1]; ++t) {
@@ -1292,7 +1308,7 @@ struct Handle {
        *w_nuy = nullptr, *w_pdf = nullptr;
   // batched solves
   int batch = 0, ncases = 1;
-  int bstep = 0;             // step index within the batch interval being launched
+  int bstep = 0;             // launch index within the batch interval being launched
   void *an_case = nullptr;   // [nan] case of each active node
   hipGraphExec_t sexec = nullptr;  // batched: graph_steps steps + advance
   hipGraph_t sgraph = nullptr;
@@ -1489,34 +1505,52 @@ static int launch_pair(Handle &h, int src) {
                                    : launch_pair_typed<float>(h, src);
 }
 
-static int launch_batch_advance(Handle &h, int n) {
+static int launch_batch_advance(Handle &h, int n, int swaps) {
   hipLaunchKernelGGL(k_batch_advance, dim3((h.ncases + 255) / 256), dim3(256), 0, h.stream,
                      h.ncases, static_cast<const int32_t *>(h.case_active),
-                     static_cast<int32_t *>(h.case_cur), static_cast<int64_t *>(h.case_steps), n);
+                     static_cast<int32_t *>(h.case_cur), static_cast<int64_t *>(h.case_steps), n,
+                     swaps);
   HIP_TRY(hipGetLastError());
   return FVDBM_OK;
 }
 
-// Batched: an interval of n steps (step index 0..n-1, the residual on the
-// last when asked) followed by the parity/step advance of active cases.
-static int launch_batch_interval(Handle &h, int n, bool resid_last) {
+// Batched: the n steps of an interval (the residual on the last when
+// asked), as two-step launches where the handle has them. Launch i runs
+// with h.bstep = i and swaps the buffers once; *swaps = launches made.
+static int launch_batch_steps(Handle &h, int n, bool resid_last, int *swaps) {
   int rc = FVDBM_OK;
-  for (int j = 0; j < n && rc == FVDBM_OK; ++j) {
-    h.bstep = j;
+  int j = 0, i = 0;
+  const int plain_end = n - (resid_last ? 1 : 0);
+  if (h.pairs)
+    for (; j + 2 <= plain_end && rc == FVDBM_OK; j += 2, ++i) {
+      h.bstep = i;
+      rc = launch_pair(h, 0);
+    }
+  for (; j < n && rc == FVDBM_OK; ++j, ++i) {
+    h.bstep = i;
     rc = launch_step_any(h, 0, resid_last && j == n - 1);
   }
   h.bstep = 0;
-  return rc ? rc : launch_batch_advance(h, n);
+  *swaps = i;
+  return rc;
 }
 
-static int launch_batch_check(Handle &h, double tol, double growth, int every) {
+// Batched: an interval of n steps (the residual on the last when asked)
+// followed by the parity/step advance of active cases.
+static int launch_batch_interval(Handle &h, int n, bool resid_last) {
+  int swaps = 0;
+  const int rc = launch_batch_steps(h, n, resid_last, &swaps);
+  return rc ? rc : launch_batch_advance(h, n, swaps);
+}
+
+static int launch_batch_check(Handle &h, double tol, double growth, int every, int swaps) {
   hipLaunchKernelGGL(k_batch_check, dim3(1), dim3(256), 0, h.stream, h.ncases,
                      static_cast<const int32_t *>(h.case_tile_off),
                      static_cast<const double *>(h.partials), tol, growth,
                      static_cast<int32_t *>(h.case_active), static_cast<int32_t *>(h.case_status),
                      static_cast<double *>(h.case_resid), static_cast<double *>(h.case_best),
                      static_cast<int32_t *>(h.n_active), static_cast<int32_t *>(h.case_cur),
-                     static_cast<int64_t *>(h.case_steps), every);
+                     static_cast<int64_t *>(h.case_steps), every, swaps);
   HIP_TRY(hipGetLastError());
   return FVDBM_OK;
 }
@@ -1535,13 +1569,9 @@ static int ensure_batch_graph(Handle &h, int every, double tol, double growth) {
     h.bgraph = nullptr;
   }
   HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
-  int rc = FVDBM_OK;
-  for (int s = 0; s < every && rc == FVDBM_OK; ++s) {
-    h.bstep = s;
-    rc = launch_step_any(h, 0, s == every - 1);
-  }
-  h.bstep = 0;
-  if (!rc) rc = launch_batch_check(h, tol, growth, every);
+  int swaps = 0;
+  int rc = launch_batch_steps(h, every, true, &swaps);
+  if (!rc) rc = launch_batch_check(h, tol, growth, every, swaps);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(h.stream, &g);
   if (rc) {
@@ -1862,8 +1892,7 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
     if (!rc) rc = h.upload(&h.cn_node, cn_node);
     if (!rc) rc = h.upload(&h.cn_d, cn_d);
   }
-  if (!rc && !(h.variant & 8) && !cell_case && h.scheme == FVDBM_UPWIND && !(h.variant & 6) &&
-      L.pair_ok) {
+  if (!rc && !(h.variant & 8) && h.scheme == FVDBM_UPWIND && !(h.variant & 6) && L.pair_ok) {
     // two-step tiles, when their LDS fits a block
     const size_t pre2 = ((size_t)Q * L.max_xnodes + kPairW * (size_t)L.max_xpairs + 7) & ~(size_t)7;
     const size_t lds2 = (pre2 + (size_t)9 * (h.block + L.max_ring)) * es;

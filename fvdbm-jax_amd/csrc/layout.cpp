@@ -643,7 +643,7 @@ bool build_layout(const fvdbm_mesh_desc &m, int reorder, int tile,
   std::copy(bs1.begin(), bs1.end(), L.bf_slot.begin() + L.nbf);
   L.bandwidth = bw;
 
-  if (!cell_case) build_pairs(m, L, pad, other, slot);
+  build_pairs(m, L, pad, other, slot);
 
   // --- faces in reference order (for the flux observable)
   L.fc0.resize(F);

@@ -48,12 +48,14 @@ def test_batch_of_one_is_single(gpu):
     assert np.array_equal(b.get_pdf(), single)
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 8])
 def test_run_residuals_and_freezing(gpu, variant):
-    """variant 2 computes boundary nodes in their own kernel, which must
-    also read each case's own buffer once the cases' parities differ."""
+    """variant 0 runs two steps per launch (two-step tiles), 8 one step per
+    launch, 2 boundary nodes in their own kernel; each must read each case's
+    own buffer once the cases' parities differ."""
     cs = _cases(3)
     b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs], variant=variant)
+    assert bool(b.info()["variant"] & 8) == (variant != 0)
     st = b.run(max_steps=200, check_every=50, tol=0.0)  # tol 0: nobody converges
     assert b.steps_run == 200 and np.all(st["steps"] == 200) and st["active"].all()
     for k, (m, dyn, c, f, n) in enumerate(cs):
