@@ -102,6 +102,14 @@ _SIGNATURES = {
     "fvdbm_nn_aggregate_backward": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
                                                ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                                ct.c_void_p]),
+    "fvdbm_nn_graph_edges": (ct.c_int64, [ct.c_int64, ct.c_int64, ct.c_int32, ct.c_int32]),
+    "fvdbm_nn_graph_error": (ct.c_char_p, []),
+    "fvdbm_nn_case_features": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                          ct.c_int32, ct.c_double, ct.c_double, ct.c_int32, ct.c_int32,
+                                          ct.c_double, ct.c_int32, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                          ct.c_void_p, ct.c_int32, ct.c_void_p]),
+    "fvdbm_nn_case_graph": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_int32,
+                                       ct.c_int32, ct.c_double] + [ct.c_void_p] * 8),
     "fvdbm_layout_order": (ct.c_int, [ct.POINTER(MeshDesc), ct.c_int32, ct.c_int32,
                                       ct.c_void_p, ct.c_void_p]),
 }

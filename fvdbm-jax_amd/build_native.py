@@ -23,6 +23,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "fvdbm_step.hip"),
     os.path.join(HERE, "csrc", "surrogate.hip"),
     os.path.join(HERE, "csrc", "layout.cpp"),
+    os.path.join(HERE, "csrc", "nn_inputs.cpp"),
 ]
 
 HIPCC_FLAGS = [

@@ -83,14 +83,15 @@ def graph_arrays(points, edges):
     return graph_from_edges(len(pts), src, dst, d[:, 0] / L, d[:, 1] / L)
 
 
-def concat_graphs(graphs):
-    """Block-diagonal union of several graph_arrays dicts."""
-    out = {k: [] for k in ("off", "col", "w", "ex", "ey", "wt", "ext", "eyt")}
+def concat_graphs(graphs, edge_keys=("w", "ex", "ey", "wt", "ext", "eyt")):
+    """Block-diagonal union of several graph_arrays dicts (with the per-edge
+    arrays ``edge_keys``)."""
+    out = {k: [] for k in ("off", "col") + tuple(edge_keys)}
     n0 = e0 = 0
     for g in graphs:
         out["off"].append(g["off"][:-1] + e0)
         out["col"].append(g["col"] + n0)
-        for k in ("w", "ex", "ey", "wt", "ext", "eyt"):
+        for k in edge_keys:
             out[k].append(g[k])
         n0 += g["n"]
         e0 += len(g["col"])
@@ -158,7 +159,10 @@ SECTION_SHIFT = np.array([0.95, 1.06, 0.5, 0.26], np.float32)
 SECTION_SCALE = np.array([0.085, 0.116, 0.31, 0.106], np.float32)
 
 
-def case_inputs(case, grid=(SLABS, 1), section=True):
+SECTION_GRID = 2048  # x resolution of porous.section_features
+
+
+def case_inputs(case, grid=(SLABS, 1)):
     """Scaled and padded node inputs [N + GX*GY + 1, 16] of a porous.Case
     (mesh nodes first, then the virtual grid nodes in row-major order, then
     the global node), its graph, and the number of mesh nodes.
@@ -167,8 +171,34 @@ def case_inputs(case, grid=(SLABS, 1), section=True):
     is linked both ways to the node of its grid cell, grid nodes to their
     8 neighbours (with the unit direction between centres), and the global
     node to every grid node. GY = 1 gives bands of x ("slabs"). Columns
-    0-8: the node features, 9-10: virtual-node flags (grid, global), 11-14
-    (with ``section``): porous.section_features."""
+    0-8: the node features, 9-10: virtual-node flags (grid, global), 11-14:
+    porous.section_features. Built by the library's host code
+    (fvdbm_nn_case_features, fvdbm_nn_case_graph), which releases the GIL,
+    so threads prepare several cases at once; ``case_inputs_numpy`` is the
+    same construction in NumPy."""
+    L = _lib.lib()
+    pts = np.ascontiguousarray(case.mesher.points, np.float64)
+    markers = np.ascontiguousarray(case.mesher.point_markers, np.int32)
+    codes = np.asarray(porous.MARKER_COLUMNS, np.int32)
+    discs = np.ascontiguousarray(np.asarray(case.discs, np.float64).reshape(-1, 3))
+    GX, GY = int(grid[0]), int(grid[1])
+    N = len(pts)
+    n = N + GX * GY + 1
+    x = np.empty((n, IN_PAD), np.float32)
+    rc = L.fvdbm_nn_case_features(_lib.ptr(pts), N, _lib.ptr(markers), _lib.ptr(codes), _lib.ptr(discs),
+                                  len(discs), float(case.u_in), float(case.dynamics.tau), GX, GY,
+                                  porous.HEIGHT, SECTION_GRID, _lib.ptr(FEATURE_SHIFT),
+                                  _lib.ptr(FEATURE_SCALE), _lib.ptr(SECTION_SHIFT),
+                                  _lib.ptr(SECTION_SCALE), IN_PAD, _lib.ptr(x))
+    if rc:
+        raise ValueError(L.fvdbm_nn_graph_error().decode())
+    g = case_graph(pts, porous.mesh_edges(case), grid)
+    return x, g, N
+
+
+def case_inputs_numpy(case, grid=(SLABS, 1)):
+    """``case_inputs`` built with NumPy (the check of the native builder;
+    features equal to float32 rounding, graph bitwise)."""
     f = porous.node_features(case)
     pts = np.asarray(case.mesher.points, np.float64)
     N = f.shape[0]
@@ -181,10 +211,9 @@ def case_inputs(case, grid=(SLABS, 1), section=True):
     gi, gj = gi.ravel(), gj.ravel()  # row-major: index = j * GX + i
     cx = (gi + 0.5) / GX
     cy = (gj + 0.5) / GY * porous.HEIGHT
-    if section:
-        vx = np.concatenate([pts[:, 0], cx, [0.5]])
-        sf = porous.section_features(np.stack([vx, np.zeros_like(vx)], 1), case.discs)
-        x[:, 11:15] = (sf - SECTION_SHIFT) / SECTION_SCALE
+    vx = np.concatenate([pts[:, 0], cx, [0.5]])
+    sf = porous.section_features(np.stack([vx, np.zeros_like(vx)], 1), case.discs, grid=SECTION_GRID)
+    x[:, 11:15] = (sf - SECTION_SHIFT) / SECTION_SCALE
     virt = np.zeros((S + 1, f.shape[1]), np.float32)
     virt[:S, 0] = cx
     virt[S, 0] = 0.5
@@ -198,10 +227,24 @@ def case_inputs(case, grid=(SLABS, 1), section=True):
     x[N:, :f.shape[1]] = (virt - FEATURE_SHIFT) / FEATURE_SCALE
     x[N:N + S, 9] = 1.0
     x[N + S, 10] = 1.0
+    g = case_graph_numpy(pts, porous.mesh_edges(case), grid)
+    return x, g, N
+
+
+def case_graph_numpy(points, edges, grid):
+    """The graph of ``case_graph`` built with NumPy (sort of the directed
+    edges by (source, target)); kept as the check of the native builder."""
+    pts = np.asarray(points, np.float64)
+    N = len(pts)
+    GX, GY = grid
+    S = GX * GY
+    n = N + S + 1
+    gi, gj = np.meshgrid(np.arange(GX), np.arange(GY), indexing="xy")
+    gi, gj = gi.ravel(), gj.ravel()
     # undirected edges: mesh faces, mesh node - its grid cell, grid cell -
     # 4 of its 8 neighbours (the other 4 are the reverses), grid - global
     us, vs, exs, eys = [], [], [], []
-    e = porous.mesh_edges(case).astype(np.int64)
+    e = np.asarray(edges).astype(np.int64)
     d = pts[e[:, 1]] - pts[e[:, 0]]
     L = np.hypot(d[:, 0], d[:, 1])
     us.append(e[:, 0]); vs.append(e[:, 1]); exs.append(d[:, 0] / L); eys.append(d[:, 1] / L)
@@ -218,8 +261,33 @@ def case_inputs(case, grid=(SLABS, 1), section=True):
     us.append(np.arange(S) + N); vs.append(np.full(S, N + S)); exs.append(np.zeros(S)); eys.append(np.zeros(S))
     src, dst, dx, dy = _interleave(np.concatenate(us), np.concatenate(vs), np.concatenate(exs),
                                    np.concatenate(eys))
-    g = graph_from_edges(n, src, dst, dx, dy, paired=True)
-    return x, g, N
+    return graph_from_edges(n, src, dst, dx, dy, paired=True)
+
+
+def case_graph(points, edges, grid):
+    """Input graph of one case (mesh nodes ``points`` [N, 2], mesh edges
+    [F, 2], virtual grid ``grid`` = (GX, GY) and the global node) as CSR
+    arrays, built by the library's host code (fvdbm_nn_case_graph; it
+    releases the GIL, so threads build several cases at once)."""
+    L = _lib.lib()
+    pts = np.ascontiguousarray(points, np.float64)
+    e = np.ascontiguousarray(edges, np.int32)
+    GX, GY = int(grid[0]), int(grid[1])
+    N = len(pts)
+    E = int(L.fvdbm_nn_graph_edges(N, len(e), GX, GY))
+    if E < 0:
+        raise ValueError("bad graph sizes")
+    n = N + GX * GY + 1
+    g = dict(off=np.empty(n + 1, np.int32), col=np.empty(E, np.int32))
+    for k in ("w", "ex", "ey", "wt", "ext", "eyt"):
+        g[k] = np.empty(E, np.float32)
+    rc = L.fvdbm_nn_case_graph(_lib.ptr(pts), N, _lib.ptr(e), len(e), GX, GY, porous.HEIGHT,
+                               *(_lib.ptr(g[k]) for k in ("off", "col", "w", "ex", "ey", "wt", "ext",
+                                                          "eyt")))
+    if rc:
+        raise ValueError(L.fvdbm_nn_graph_error().decode())
+    g["n"] = n
+    return g
 
 
 # ---------------------------------------------------------------------------
@@ -408,16 +476,17 @@ class MeshNet:
         """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
         with ``project``, u_x passes through ``project_mass_flux``. Cases
         go through the network ``chunk`` at a time; their inputs are
-        prepared by a thread pool (NumPy releases the GIL in the sorts)
+        prepared by a thread pool (the native builders release the GIL)
         while the previous chunk runs on the GPU."""
         torch = _torch()
+        import os
         from concurrent.futures import ThreadPoolExecutor
         out = []
-        with ThreadPoolExecutor(max_workers=8) as pool:
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
             futs = [pool.submit(case_inputs, c, self.grid) for c in cases]
             for c0 in range(0, len(cases), chunk):
                 ins = [f.result() for f in futs[c0:c0 + chunk]]
-                g = graph_to_device(concat_graphs([i[1] for i in ins]))
+                g = graph_to_device(concat_graphs([i[1] for i in ins], edge_keys=("w", "ex", "ey")))
                 x = torch.from_numpy(np.concatenate([i[0] for i in ins])).cuda()
                 with torch.no_grad():
                     y = self.forward(x, g).cpu().numpy().astype(np.float64) * self.target_scale

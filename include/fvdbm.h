@@ -257,6 +257,38 @@ int fvdbm_nn_aggregate_backward(const float *dz, int64_t n_nodes, const int32_t 
                                 const int32_t *col, const float *wt, const float *ext,
                                 const float *eyt, float *dh, void *stream);
 
+/* Input graph of one porous case, on the host (host pointers; thread-safe,
+ * no device work): mesh nodes xy [n_mesh][2] and mesh edges [n_edges][2],
+ * plus a virtual gx x gy grid over [0,1] x [0,height] (nodes n_mesh ..,
+ * row-major) and a global node (last), n = n_mesh + gx*gy + 1 nodes. Edges,
+ * each stored both ways: mesh edges (unit direction), mesh node <-> its grid
+ * cell (direction 0), grid node <-> its 8 neighbours (unit direction between
+ * centres), grid node <-> global node (direction 0). CSR sorted by (source,
+ * target): row_off [n+1], col [E], w = 1/deg(source), ex, ey, and the
+ * reverse edge's wt = 1/deg(target), ext, eyt, with
+ * E = fvdbm_nn_graph_edges(n_mesh, n_edges, gx, gy) (-1 for bad sizes).
+ * Errors: FVDBM_ERR_INVALID, message in fvdbm_nn_graph_error(). */
+int64_t fvdbm_nn_graph_edges(int64_t n_mesh, int64_t n_edges, int32_t gx, int32_t gy);
+const char *fvdbm_nn_graph_error(void);
+/* Node inputs of one porous case, on the host, rows in the node order
+ * above, x [n][width] (width >= 16): columns 0-8 = (x, y, signed distance
+ * to the nearest disc, marker == marker_codes[0..3], u_in, tau), each as
+ * (float(v) - shift[c]) / scale[c]; virtual nodes take the cell centre
+ * (the global node (0.5, 0.25)), y = 0.25 and distance shift[2] when gy = 1,
+ * and flag column 9 (grid) or 10 (global); columns 11-14 = the channel's
+ * cross-section at the node's x on a section_grid-point grid (open
+ * fraction a, 1/a, share of the total resistance sum 1/a^3 downstream, log
+ * of that total), as (v - sshift) / sscale; other columns 0. discs
+ * [n_discs][3] = (cx, cy, r). */
+int fvdbm_nn_case_features(const double *xy, int64_t n_mesh, const int32_t *markers,
+                           const int32_t *marker_codes, const double *discs, int32_t n_discs,
+                           double u_in, double tau, int32_t gx, int32_t gy, double height,
+                           int32_t section_grid, const float *shift, const float *scale,
+                           const float *sshift, const float *sscale, int32_t width, float *x);
+int fvdbm_nn_case_graph(const double *xy, int64_t n_mesh, const int32_t *edges, int64_t n_edges,
+                        int32_t gx, int32_t gy, double height, int32_t *row_off, int32_t *col,
+                        float *w, float *ex, float *ey, float *wt, float *ext, float *eyt);
+
 #ifdef __cplusplus
 }
 #endif

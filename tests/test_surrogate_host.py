@@ -62,6 +62,30 @@ def test_virtual_nodes():
         assert g["wt"][i] == g["w"][r] and g["ext"][i] == g["ex"][r] and g["eyt"][i] == g["ey"][r]
 
 
+def test_native_inputs_match_numpy():
+    """The library's host builders of the surrogate inputs
+    (fvdbm_nn_case_features, fvdbm_nn_case_graph) against the NumPy
+    construction: graph arrays bitwise, features to float32 rounding (the
+    NumPy total resistance is a pairwise sum, the native one sequential)."""
+    for seed, cells in ((2, 1200), (5, 3000)):
+        c = fv.porous.make_case(seed, target_cells=cells)
+        for grid in ((32, 16), (32, 1), (5, 3)):
+            x, g, N = sg.case_inputs(c, grid)
+            x2, g2, N2 = sg.case_inputs_numpy(c, grid)
+            assert N == N2 and g["n"] == g2["n"]
+            np.testing.assert_allclose(x, x2, rtol=1e-6, atol=1e-6)
+            for k in ("off", "col", "w", "ex", "ey", "wt", "ext", "eyt"):
+                assert g[k].dtype == g2[k].dtype and np.array_equal(g[k], g2[k]), (grid, k)
+    bad = np.array(c.mesher.faces, np.int32)
+    bad[3, 1] = c.n_nodes  # out of range
+    try:
+        sg.case_graph(c.mesher.points, bad, (4, 2))
+    except ValueError as e:
+        assert "out of range" in str(e)
+    else:
+        raise AssertionError("an out-of-range mesh edge was accepted")
+
+
 def test_twin_aggregate_matches_dense_adjacency():
     c, g = _case()
     N = c.n_nodes
