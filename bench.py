@@ -14,8 +14,8 @@ solver's stream (launched as captured hipGraphs), with the inputs resident
 in HBM. value = cells * K / elapsed / 1e6 (MLUPS). roofline.achieved = the
 algorithmic 108 B per cell update * cells * 2 per k_step2 launch / the
 launch's duration, i.e. 108 B/LUP * cells / (elapsed/K).
-cpu_baseline = the C oracle (scalar, 1 thread) on a bounded number of steps
-of the same mesh, on this host.
+cpu_baseline = the C oracle (OpenMP over the element loops, min(16, CPUs)
+threads) on a bounded number of steps of the same mesh, on this host.
 
 batched: ``--batch-cases`` (default 256) porous cases of the batched
 workload (5-40 discs, 20k-200k cells, seeds 0..) in one handle: the
@@ -94,10 +94,13 @@ def build_case(cells, seed, scheme="upwind", cache_dir=None):
     return fv, m, dyn, c, f, n, dict(dim_multiplier=mult, dt=dyn.delta_t, tau=tau)
 
 
-def cpu_baseline(m, dyn, c, f, n, budget_s=12.0):
-    """The oracle (scalar C, 1 thread) on a bounded number of steps."""
+def cpu_baseline(m, dyn, c, f, n, budget_s=12.0, threads=None):
+    """The oracle (C, element loops split over ``threads`` OpenMP threads,
+    default min(16, CPUs): the host share of one GPU) on a bounded number
+    of steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import oracle
+    threads = oracle.set_threads(threads or min(16, os.cpu_count() or 1))
     env = dict(cell_faces=c.face_indices, cell_signs=c.face_normals,
                face_cells=f.stencil_cells_index, face_dists=f.stencil_dists,
                face_nodes=f.nodes_index, face_n=f.n, face_L=np.asarray(f.L).reshape(-1),
@@ -111,9 +114,9 @@ def cpu_baseline(m, dyn, c, f, n, budget_s=12.0):
     oracle.run(env, st, dyn.tau, dyn.delta_t, steps, "f32")
     el = time.perf_counter() - t0
     C = c.face_indices.shape[0]
-    return dict(value=round(C * steps / el / 1e6, 3), unit="MLUPS", cores=1, kind="port",
+    return dict(value=round(C * steps / el / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
                 sample=f"{steps} steps of the same {C}-cell mesh, FP32, oracle C restatement "
-                       f"(oracle/fvdbm_oracle.c, gcc -O2), 1 thread, {el:.1f}s")
+                       f"(oracle/fvdbm_oracle.c, gcc -O2 -fopenmp), {threads} threads, {el:.1f}s")
 
 
 WEIGHTS = os.path.join(ROOT, "fvdbm-jax_amd", "data", "surrogate_porous.npz")

@@ -33,6 +33,20 @@
 #include <math.h>
 #include <stdint.h>
 #include <stddef.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the element loops (1 without OpenMP); n < 1 leaves it. */
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n >= 1) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 #define Q 9
 

@@ -73,6 +73,15 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """Threads of the oracle's element loops (OpenMP); returns the count in
+    use. The results do not depend on it."""
+    fn = lib().oracle_set_threads
+    fn.restype = ct.c_int
+    fn.argtypes = [ct.c_int]
+    return int(fn(int(n)))
+
+
 def _ptr(a):
     return a.ctypes.data_as(ct.c_void_p)
 
