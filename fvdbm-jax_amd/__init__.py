@@ -16,11 +16,11 @@ import sys as _sys
 
 from . import autotune, meshgen, porous
 from .containers import Cells, CustomArray, Faces, Nodes
-from .dynamics import D2Q9, Dynamics
+from .dynamics import D2Q9, D2Q13, Dynamics
 from .environment import Environment
 from .mesher import Mesher
 
 _sys.modules.setdefault("fvdbm_jax_amd", _sys.modules[__name__])
 
-__all__ = ["Cells", "CustomArray", "D2Q9", "Dynamics", "Environment", "Faces",
+__all__ = ["Cells", "CustomArray", "D2Q9", "D2Q13", "Dynamics", "Environment", "Faces",
            "Mesher", "Nodes", "autotune", "meshgen", "porous"]

@@ -76,6 +76,17 @@ _SIGNATURES = {
     "fvdbm_synchronize": (ct.c_int, [ct.c_void_p]),
     "fvdbm_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "fvdbm_copy_probe": (ct.c_int, [ct.c_int64, ct.c_int32, ct.POINTER(ct.c_double)]),
+    "fvdbm_lattice_create": (ct.c_int, [ct.POINTER(MeshDesc), ct.POINTER(Params), ct.c_int32,
+                                        ct.POINTER(ct.c_void_p)]),
+    "fvdbm_lattice_destroy": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_lattice_init_uniform": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_lattice_step": (ct.c_int, [ct.c_void_p, ct.c_int64]),
+    "fvdbm_lattice_step_timed": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
+    "fvdbm_lattice_synchronize": (ct.c_int, [ct.c_void_p]),
+    "fvdbm_lattice_get_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),
+    "fvdbm_lattice_set_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),
+    "fvdbm_lattice_info_get": (ct.c_int, [ct.c_void_p, ct.POINTER(Info)]),
+    "fvdbm_lattice_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "fvdbm_create_batch": (ct.c_int, [ct.POINTER(MeshDesc), ct.POINTER(Params), ct.POINTER(BatchDesc),
                                       ct.POINTER(ct.c_void_p)]),
     "fvdbm_batch_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_double, ct.c_double,

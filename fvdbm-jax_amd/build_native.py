@@ -21,6 +21,7 @@ LIB = os.path.join(BUILD, "libfvdbm.so")
 
 SOURCES = [
     os.path.join(HERE, "csrc", "fvdbm_step.hip"),
+    os.path.join(HERE, "csrc", "fvdbm_lattice.hip"),
     os.path.join(HERE, "csrc", "surrogate.hip"),
     os.path.join(HERE, "csrc", "layout.cpp"),
     os.path.join(HERE, "csrc", "nn_inputs.cpp"),

@@ -1,0 +1,905 @@
+// FVDBM step for any supported lattice: D2Q13 (src/dynamics.py:77-102), and
+// D2Q9 through the same code as a cross-check of the tuned D2Q9 kernels in
+// fvdbm_step.hip. One step is two plain kernels: the active boundary nodes
+// (Nodes.calc_pdfs, src/containers.py:339-390) into the node pdf array, then
+// one thread per cell for the face fluxes and the BGK update
+// (src/containers.py:107-121, 191-287). Same reference operation order and
+// no contraction (-ffp-contract=off) as the tuned kernels, so D2Q9 results
+// equal theirs bit for bit and D2Q13 results equal the C oracle's.
+//
+// Layout (device cell order and every per-cell record) comes from the same
+// host builder (layout.cpp); pdfs are q-major [Q][Cp], node pdfs [Q][N].
+// C ABI: fvdbm_lattice_* in include/fvdbm.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "errors.h"
+#include "fvdbm.h"
+#include "layout.h"
+
+namespace fvdbm_lattice {
+
+using fvdbm::HostLayout;
+using fvdbm::kIndexMask;
+using fvdbm::kSlot1Bit;
+
+static int fail(int code, const std::string &m) { return fvdbm::set_error(code, m); }
+
+#define LHIP_TRY(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(FVDBM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));     \
+  } while (0)
+
+// Velocities (src/dynamics.py:54-62 and 80-96): D2Q13 is D2Q9 plus
+// q = 9..12 = (2,0), (0,2), (-2,0), (0,-2).
+__host__ __device__ constexpr int kx(int q) {
+  return q == 1 || q == 5 || q == 8 ? 1
+         : q == 3 || q == 6 || q == 7 ? -1
+         : q == 9                     ? 2
+         : q == 11                    ? -2
+                                      : 0;
+}
+__host__ __device__ constexpr int ky(int q) {
+  return q == 2 || q == 5 || q == 6 ? 1
+         : q == 4 || q == 7 || q == 8 ? -1
+         : q == 10                    ? 2
+         : q == 12                    ? -2
+                                      : 0;
+}
+
+// W rounded from the Python doubles; C = 1/sqrt(3) (D2Q9) or 1/sqrt(2)
+// (D2Q13) in the working precision; powers of C as lax.integer_pow forms
+// them: C^2 = C*C, C^4 = C^2*C^2, C^6 = C^2*C^4.
+template <typename T, int QN>
+struct Consts {
+  T W[QN];
+  T C2, C4x2, C2x2, C6x2;  // C^2, 2 C^4, 2 C^2, 2 C^6
+  T invtau, dt;            // 1/tau rounded from double, delta_t rounded
+};
+
+template <typename T, int QN>
+static Consts<T, QN> make_consts(double tau, double dt) {
+  static const double w9[9] = {4.0 / 9,  1.0 / 9,  1.0 / 9,  1.0 / 9, 1.0 / 9,
+                               1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+  static const double w13[13] = {3.0 / 8,  1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12,
+                                 1.0 / 16, 1.0 / 16, 1.0 / 16, 1.0 / 16, 1.0 / 96,
+                                 1.0 / 96, 1.0 / 96, 1.0 / 96};
+  Consts<T, QN> k;
+  for (int q = 0; q < QN; ++q) k.W[q] = T(QN == 13 ? w13[q] : w9[q]);
+  const T C = T(1) / std::sqrt(T(QN == 13 ? 2 : 3));
+  k.C2 = C * C;
+  const T c4 = k.C2 * k.C2;
+  const T c6 = k.C2 * c4;
+  k.C4x2 = T(2) * c4;
+  k.C2x2 = T(2) * k.C2;
+  k.C6x2 = T(2) * c6;
+  k.invtau = T(1.0 / tau);
+  k.dt = T(dt);
+  return k;
+}
+
+// rho = sum_q f_q; u = (sum_q ksi_q f_q) / rho (0-initialised, q order)
+template <typename T, int QN>
+__device__ __forceinline__ void macro(const T (&f)[QN], T &rho, T &ux, T &uy) {
+  T r = T(0);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) r += f[q];
+  T mx = T(0), my = T(0);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    mx += T(kx(q)) * f[q];
+    my += T(ky(q)) * f[q];
+  }
+  rho = r;
+  ux = mx / r;
+  uy = my / r;
+}
+
+// (W rho) * ((((1 + cu/C^2) + cu^2/(2C^4)) - uu/(2C^2)) [D2Q13: + cu^3/(2C^6)
+// - 3 cu uu/(2C^4)]), cu = ksi.u and uu = u.u as 0-initialised sums,
+// cu^3 = cu*(cu*cu) (src/dynamics.py:70-74, 101-102)
+template <typename T, int QN>
+__device__ __forceinline__ void equilibrium(const Consts<T, QN> &k, T rho, T ux, T uy,
+                                            T (&out)[QN]) {
+  T uu = T(0);
+  uu += ux * ux;
+  uu += uy * uy;
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    T cu = T(0);
+    cu += T(kx(q)) * ux;
+    cu += T(ky(q)) * uy;
+    T br = ((T(1) + cu / k.C2) + (cu * cu) / k.C4x2) - uu / k.C2x2;
+    if constexpr (QN == 13) br = (br + (cu * (cu * cu)) / k.C6x2) - ((T(3) * cu) * uu) / k.C4x2;
+    out[q] = (k.W[q] * rho) * br;
+  }
+}
+
+template <typename T>
+struct Args {
+  const T *fsrc;
+  T *fdst;
+  int C, Cp, N, K, nbf, nan;
+  const int32_t *code;  // [3][Cp] neighbour device id | slot-1 bit, or ghost bit | record
+  const T *nx, *ny, *sL, *lwa, *lwden2;  // [3][Cp]
+  const int32_t *bf_gnode;               // [2][nbf] face-node ids of each ghost face
+  const T *bf_ratio;                     // [nbf] d_ghost / d_known
+  T *node_pdf, *node_rho, *node_vel;     // [Q][N], [N], [2][N]
+  const int32_t *an_gid, *an_type, *an_cells;
+  const T *an_w, *an_wsum, *an_rho, *an_ux, *an_uy;
+};
+
+template <typename T, int QN>
+__device__ __forceinline__ void load_cell(const T *f, size_t Cp, int c, T (&out)[QN]) {
+#pragma unroll
+  for (int q = 0; q < QN; ++q) out[q] = f[(size_t)q * Cp + c];
+}
+
+// Active nodes from fsrc: type 1 (velocity) rho <- IDW(cell rho), type 2
+// (density) u <- IDW(cell u); pdf = calc_eq(rho, u) + IDW(f_c - feq_c).
+// Padding entries (weight 0) contribute +0, as x*0 does in the reference.
+template <typename T, int QN>
+__global__ void k_lat_nodes(Args<T> a, Consts<T, QN> k) {
+  const int an = blockIdx.x * blockDim.x + threadIdx.x;
+  if (an >= a.nan) return;
+  const int type = a.an_type[an];
+  const int32_t *cells = a.an_cells + (size_t)an * a.K;
+  const T *w = a.an_w + (size_t)an * a.K;
+  const T wsum = a.an_wsum[an];
+  T rho_n = a.an_rho[an], ux_n = a.an_ux[an], uy_n = a.an_uy[an];
+  T n0 = T(0), n1 = T(0);
+  for (int j = 0; j < a.K; ++j) {
+    const int cj = cells[j];
+    const T wj = w[j];
+    T r = T(0), ux = T(0), uy = T(0);
+    if (!(cj < 0 && wj == T(0))) {
+      T fc[QN];
+      load_cell<T, QN>(a.fsrc, a.Cp, cj < 0 ? -2 - cj : cj, fc);
+      macro<T, QN>(fc, r, ux, uy);
+    }
+    if (type == 1) {
+      n0 += r * wj;
+    } else {
+      n0 += ux * wj;
+      n1 += uy * wj;
+    }
+  }
+  if (type == 1) {
+    rho_n = n0 / wsum;
+  } else {
+    ux_n = n0 / wsum;
+    uy_n = n1 / wsum;
+  }
+  T acc[QN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) acc[q] = T(0);
+  for (int j = 0; j < a.K; ++j) {
+    const int cj = cells[j];
+    const T wj = w[j];
+    if (cj < 0) {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) acc[q] += T(0) * wj;
+    } else {
+      T fc[QN], eq[QN], r, ux, uy;
+      load_cell<T, QN>(a.fsrc, a.Cp, cj, fc);
+      macro<T, QN>(fc, r, ux, uy);
+      equilibrium<T, QN>(k, r, ux, uy, eq);
+#pragma unroll
+      for (int q = 0; q < QN; ++q) acc[q] += (fc[q] - eq[q]) * wj;
+    }
+  }
+  T eqn[QN];
+  equilibrium<T, QN>(k, rho_n, ux_n, uy_n, eqn);
+  const int g = a.an_gid[an];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) a.node_pdf[(size_t)q * a.N + g] = eqn[q] + acc[q] / wsum;
+  if (type == 1) {
+    a.node_rho[g] = rho_n;
+  } else {
+    a.node_vel[g] = ux_n;
+    a.node_vel[(size_t)a.N + g] = uy_n;
+  }
+}
+
+// One cell per thread: the three faces in cells.face_indices order (ghost
+// g = fbar + (fbar - f_self) * ratio from the two face-node pdfs), upwind
+// by the sign of ksi.n (SCHEME 0) or Lax-Wendroff with slot 0 upstream
+// (SCHEME 1), flux (pdf*phi)*(s*L) summed in face order, then
+// f + dt*(1/tau*(feq - f) - sum) into fdst.
+template <typename T, int QN, int SCHEME>
+__global__ void k_lat_step(Args<T> a, Consts<T, QN> k) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.C) return;
+  const size_t Cp = a.Cp;
+  T f[QN], acc[QN];
+  load_cell<T, QN>(a.fsrc, Cp, c, f);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) acc[q] = T(0);
+  for (int kf = 0; kf < 3; ++kf) {
+    const size_t at = (size_t)kf * Cp + c;
+    const int32_t code = a.code[at];
+    const T nx = a.nx[at], ny = a.ny[at], sL = a.sL[at];
+    const bool slot1 = (code & kSlot1Bit) != 0;
+    T o[QN];
+    if (code >= 0) {
+      load_cell<T, QN>(a.fsrc, Cp, code & kIndexMask, o);
+    } else {
+      const int bf = code & kIndexMask;
+      const T ratio = a.bf_ratio[bf];
+      const int g0 = a.bf_gnode[bf], g1 = a.bf_gnode[a.nbf + bf];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const T fbar = (a.node_pdf[(size_t)q * a.N + g0] + a.node_pdf[(size_t)q * a.N + g1]) / T(2);
+        o[q] = fbar + (fbar - f[q]) * ratio;
+      }
+    }
+    T la = T(0), den2 = T(1);
+    if constexpr (SCHEME == 1) {
+      la = a.lwa[at];
+      den2 = a.lwden2[at];
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      T phi = T(0);
+      phi += T(kx(q)) * nx;
+      phi += T(ky(q)) * ny;
+      if constexpr (SCHEME == 0) {
+        const bool take_slot0 = phi >= T(0);
+        const T up = (take_slot0 != slot1) ? f[q] : o[q];
+        acc[q] += (up * phi) * sL;
+      } else {
+        const T fu = slot1 ? o[q] : f[q];
+        const T fd = slot1 ? f[q] : o[q];
+        const T interp = la - (phi * k.dt) / den2;
+        const T fcv = fu + (fd - fu) * interp;
+        acc[q] += (fcv * phi) * sL;
+      }
+    }
+  }
+  T rho, ux, uy, eq[QN];
+  macro<T, QN>(f, rho, ux, uy);
+  equilibrium<T, QN>(k, rho, ux, uy, eq);
+#pragma unroll
+  for (int q = 0; q < QN; ++q)
+    a.fdst[(size_t)q * Cp + c] = f[q] + k.dt * (k.invtau * (eq[q] - f[q]) - acc[q]);
+}
+
+// cells.rho / vel / pdf_eq of fsrc (device order)
+template <typename T, int QN>
+__global__ void k_lat_moments(Args<T> a, Consts<T, QN> k, T *rho, T *vel, T *feq) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.C) return;
+  T f[QN], eq[QN], r, ux, uy;
+  load_cell<T, QN>(a.fsrc, a.Cp, c, f);
+  macro<T, QN>(f, r, ux, uy);
+  equilibrium<T, QN>(k, r, ux, uy, eq);
+  rho[c] = r;
+  vel[c] = ux;
+  vel[(size_t)a.Cp + c] = uy;
+#pragma unroll
+  for (int q = 0; q < QN; ++q) feq[(size_t)q * a.Cp + c] = eq[q];
+}
+
+// faces.pdf in reference face order from fsrc and the node pdfs
+// (calc_upwind_flux / calc_lax_wendroff_flux face by face, ghosts on
+// either slot, index -1 of a missing known cell read as the last cell)
+struct FaceArrays {
+  const int32_t *c0, *c1, *n0, *n1;
+  const void *d0, *d1, *nx, *ny, *L;
+  int F, last_cell;
+};
+
+template <typename T, int QN, int SCHEME>
+__global__ void k_lat_flux(Args<T> a, Consts<T, QN> k, FaceArrays fa, T *flux) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= fa.F) return;
+  const int ia = fa.c0[j], ib = fa.c1[j];
+  const T d0 = static_cast<const T *>(fa.d0)[j], d1 = static_cast<const T *>(fa.d1)[j];
+  const T nx = static_cast<const T *>(fa.nx)[j], ny = static_cast<const T *>(fa.ny)[j];
+  const T L = static_cast<const T *>(fa.L)[j];
+  T fl[QN], fr[QN];
+  for (int side = 0; side < 2; ++side) {
+    const int idx = side == 0 ? ia : ib;
+    T tmp[QN];
+    if (idx != -1) {
+      load_cell<T, QN>(a.fsrc, a.Cp, idx, tmp);
+    } else {
+      int known = side == 0 ? ib : ia;
+      if (known < 0) known = fa.last_cell;
+      const T gd = side == 0 ? d0 : d1, kd = side == 0 ? d1 : d0;
+      const T ratio = gd / kd;
+      T fk[QN];
+      load_cell<T, QN>(a.fsrc, a.Cp, known, fk);
+      const int p = fa.n0[j], r = fa.n1[j];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const T fbar = (a.node_pdf[(size_t)q * a.N + p] + a.node_pdf[(size_t)q * a.N + r]) / T(2);
+        tmp[q] = fbar + (fbar - fk[q]) * ratio;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) (side == 0 ? fl : fr)[q] = tmp[q];
+  }
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    T phi = T(0);
+    phi += T(kx(q)) * nx;
+    phi += T(ky(q)) * ny;
+    T v;
+    if constexpr (SCHEME == 0) {
+      v = ((phi >= T(0)) ? fl[q] : fr[q]) * phi;
+    } else {
+      const T s = d0 + d1;
+      const T interp = (d0 / s) - (phi * k.dt) / (T(2) * s);
+      v = (fl[q] + (fr[q] - fl[q]) * interp) * phi;
+    }
+    flux[(size_t)q * fa.F + j] = v * L;
+  }
+}
+
+// rows [0, n) of a q-major [Q][stride] array <- W
+template <typename T, int QN>
+__global__ void k_lat_fill(T *f, int n, int stride, Consts<T, QN> k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < QN; ++q) f[(size_t)q * stride + i] = k.W[q];
+}
+
+// ----------------------------------------------------------------------------
+// handle
+// ----------------------------------------------------------------------------
+struct Handle {
+  int q = 13, precision = FVDBM_FP32, scheme = FVDBM_UPWIND, block = 256, graph_steps = 32;
+  double tau = 1.0, dt = 1.0;
+  int C = 0, Cp = 0, F = 0, N = 0, K = 0, nbf = 0, nan = 0, nbc = 0, bandwidth = 0,
+      last_cell_new = 0;
+  std::vector<int32_t> new_to_old, an_gid_host;
+  void *f[2] = {nullptr, nullptr};
+  void *code = nullptr, *nx = nullptr, *ny = nullptr, *sL = nullptr, *lwa = nullptr,
+       *lwden2 = nullptr, *bf_gnode = nullptr, *bf_ratio = nullptr;
+  void *node_pdf = nullptr, *node_rho = nullptr, *node_vel = nullptr;
+  void *an_gid = nullptr, *an_type = nullptr, *an_cells = nullptr, *an_w = nullptr,
+       *an_wsum = nullptr, *an_rho = nullptr, *an_ux = nullptr, *an_uy = nullptr;
+  void *fc0 = nullptr, *fc1 = nullptr, *fn0 = nullptr, *fn1 = nullptr, *fd0 = nullptr,
+       *fd1 = nullptr, *fnx = nullptr, *fny = nullptr, *fL = nullptr;
+  void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
+  int cur = 0;
+  bool has_prev = false, observed = false;
+  int64_t steps = 0, device_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<void *> bufs;
+
+  ~Handle() {
+    for (int i = 0; i < 2; ++i) {
+      if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    }
+    for (void *b : bufs) (void)hipFree(b);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int alloc(void **out, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(FVDBM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bufs.push_back(p);
+    device_bytes += (int64_t)bytes;
+    *out = p;
+    return FVDBM_OK;
+  }
+  template <typename V>
+  int upload(void **out, const std::vector<V> &v) {
+    int rc = alloc(out, v.size() * sizeof(V));
+    if (rc) return rc;
+    if (!v.empty()) LHIP_TRY(hipMemcpy(*out, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    return FVDBM_OK;
+  }
+};
+
+template <typename T>
+static Args<T> make_args(const Handle &h, int src) {
+  Args<T> a;
+  a.fsrc = static_cast<const T *>(h.f[src]);
+  a.fdst = static_cast<T *>(h.f[src ^ 1]);
+  a.C = h.C;
+  a.Cp = h.Cp;
+  a.N = h.N;
+  a.K = h.K;
+  a.nbf = h.nbf;
+  a.nan = h.nan;
+  a.code = static_cast<const int32_t *>(h.code);
+  a.nx = static_cast<const T *>(h.nx);
+  a.ny = static_cast<const T *>(h.ny);
+  a.sL = static_cast<const T *>(h.sL);
+  a.lwa = static_cast<const T *>(h.lwa);
+  a.lwden2 = static_cast<const T *>(h.lwden2);
+  a.bf_gnode = static_cast<const int32_t *>(h.bf_gnode);
+  a.bf_ratio = static_cast<const T *>(h.bf_ratio);
+  a.node_pdf = static_cast<T *>(h.node_pdf);
+  a.node_rho = static_cast<T *>(h.node_rho);
+  a.node_vel = static_cast<T *>(h.node_vel);
+  a.an_gid = static_cast<const int32_t *>(h.an_gid);
+  a.an_type = static_cast<const int32_t *>(h.an_type);
+  a.an_cells = static_cast<const int32_t *>(h.an_cells);
+  a.an_w = static_cast<const T *>(h.an_w);
+  a.an_wsum = static_cast<const T *>(h.an_wsum);
+  a.an_rho = static_cast<const T *>(h.an_rho);
+  a.an_ux = static_cast<const T *>(h.an_ux);
+  a.an_uy = static_cast<const T *>(h.an_uy);
+  return a;
+}
+
+static inline dim3 grid1(int n) { return dim3((unsigned)std::max(1, (n + 255) / 256)); }
+
+// one step reading f[src]: the active nodes, then the cells
+template <typename T, int QN>
+static int launch_step(Handle &h, int src) {
+  const Args<T> a = make_args<T>(h, src);
+  const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+  if (h.nan > 0) hipLaunchKernelGGL((k_lat_nodes<T, QN>), grid1(h.nan), dim3(256), 0, h.stream, a, k);
+  const dim3 grid((unsigned)std::max(1, (h.C + h.block - 1) / h.block));
+  if (h.scheme == FVDBM_UPWIND)
+    hipLaunchKernelGGL((k_lat_step<T, QN, 0>), grid, dim3(h.block), 0, h.stream, a, k);
+  else
+    hipLaunchKernelGGL((k_lat_step<T, QN, 1>), grid, dim3(h.block), 0, h.stream, a, k);
+  LHIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+template <typename F>
+static int dispatch(const Handle &h, F &&fn) {
+  if (h.precision == FVDBM_FP64) return h.q == 13 ? fn((double *)nullptr, std::integral_constant<int, 13>())
+                                                   : fn((double *)nullptr, std::integral_constant<int, 9>());
+  return h.q == 13 ? fn((float *)nullptr, std::integral_constant<int, 13>())
+                   : fn((float *)nullptr, std::integral_constant<int, 9>());
+}
+
+static int step_any(Handle &h, int src) {
+  return dispatch(h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_step<T, decltype(qn)::value>(h, src);
+  });
+}
+
+static int ensure_graph(Handle &h, int parity) {
+  if (h.gexec[parity]) return FVDBM_OK;
+  LHIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  int src = parity, rc = FVDBM_OK;
+  for (int s = 0; s < h.graph_steps && rc == FVDBM_OK; ++s) {
+    rc = step_any(h, src);
+    src ^= 1;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h.graph[parity] = g;
+  LHIP_TRY(hipGraphInstantiate(&h.gexec[parity], g, nullptr, nullptr, 0));
+  return FVDBM_OK;
+}
+
+static int run_steps(Handle &h, int64_t n) {
+  if (n <= 0) return FVDBM_OK;
+  if (h.graph_steps > 0) {
+    while (n >= h.graph_steps) {
+      int rc = ensure_graph(h, h.cur);
+      if (rc) return rc;
+      LHIP_TRY(hipGraphLaunch(h.gexec[h.cur], h.stream));
+      n -= h.graph_steps;  // graph_steps is even: parity unchanged
+      h.steps += h.graph_steps;
+    }
+  }
+  for (; n > 0; --n) {
+    int rc = step_any(h, h.cur);
+    if (rc) return rc;
+    h.cur ^= 1;
+    ++h.steps;
+  }
+  h.has_prev = true;
+  h.observed = false;
+  return FVDBM_OK;
+}
+
+template <typename T, int QN>
+static int init_uniform(Handle &h) {
+  const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+  for (int i = 0; i < 2; ++i)
+    hipLaunchKernelGGL((k_lat_fill<T, QN>), grid1(h.Cp), dim3(256), 0, h.stream,
+                       static_cast<T *>(h.f[i]), h.Cp, h.Cp, k);
+  hipLaunchKernelGGL((k_lat_fill<T, QN>), grid1(h.N), dim3(256), 0, h.stream,
+                     static_cast<T *>(h.node_pdf), h.N, h.N, k);
+  LHIP_TRY(hipGetLastError());
+  LHIP_TRY(hipStreamSynchronize(h.stream));
+  h.cur = 0;
+  h.steps = 0;
+  h.has_prev = false;
+  h.observed = false;
+  return FVDBM_OK;
+}
+
+template <typename T, int QN>
+static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
+  HostLayout<T> L;
+  std::string err;
+  if (!fvdbm::build_layout<T>(m, reorder, h.block, L, err)) return fail(FVDBM_ERR_INVALID, err);
+  h.C = L.C;
+  h.Cp = L.Cp;
+  h.F = L.F;
+  h.N = L.N;
+  h.K = L.K;
+  h.nbf = L.nbf;
+  h.nan = L.nan;
+  h.nbc = L.n_bcells;
+  h.bandwidth = L.bandwidth;
+  h.last_cell_new = L.last_cell_new;
+  h.new_to_old = L.new_to_old;
+  h.an_gid_host = L.an_gid;
+  const size_t es = sizeof(T);
+  int rc = FVDBM_OK;
+  for (int i = 0; i < 2 && !rc; ++i) rc = h.alloc(&h.f[i], (size_t)QN * h.Cp * es);
+  if (!rc) rc = h.upload(&h.code, L.nb);
+  if (!rc) rc = h.upload(&h.nx, L.nx);
+  if (!rc) rc = h.upload(&h.ny, L.ny);
+  if (!rc) rc = h.upload(&h.sL, L.sL);
+  if (!rc && h.scheme == FVDBM_LAX_WENDROFF) rc = h.upload(&h.lwa, L.lwa);
+  if (!rc && h.scheme == FVDBM_LAX_WENDROFF) rc = h.upload(&h.lwden2, L.lwden2);
+  if (!rc) rc = h.upload(&h.bf_gnode, L.bf_gnode);
+  if (!rc) rc = h.upload(&h.bf_ratio, L.bf_ratio);
+  if (!rc) rc = h.alloc(&h.node_pdf, (size_t)QN * h.N * es);
+  if (!rc) rc = h.alloc(&h.node_rho, (size_t)h.N * es);
+  if (!rc) rc = h.alloc(&h.node_vel, (size_t)2 * h.N * es);
+  if (!rc) rc = h.upload(&h.an_gid, L.an_gid);
+  if (!rc) rc = h.upload(&h.an_type, L.an_type);
+  if (!rc) rc = h.upload(&h.an_cells, L.an_cells);
+  if (!rc) rc = h.upload(&h.an_w, L.an_w);
+  if (!rc) rc = h.upload(&h.an_wsum, L.an_wsum);
+  if (!rc) rc = h.upload(&h.an_rho, L.an_rho);
+  if (!rc) rc = h.upload(&h.an_ux, L.an_ux);
+  if (!rc) rc = h.upload(&h.an_uy, L.an_uy);
+  if (!rc) rc = h.upload(&h.fc0, L.fc0);
+  if (!rc) rc = h.upload(&h.fc1, L.fc1);
+  if (!rc) rc = h.upload(&h.fn0, L.fn0);
+  if (!rc) rc = h.upload(&h.fn1, L.fn1);
+  if (!rc) rc = h.upload(&h.fd0, L.fd0);
+  if (!rc) rc = h.upload(&h.fd1, L.fd1);
+  if (!rc) rc = h.upload(&h.fnx, L.fnx);
+  if (!rc) rc = h.upload(&h.fny, L.fny);
+  if (!rc) rc = h.upload(&h.fL, L.fL);
+  if (rc) return rc;
+  // node rho/vel hold the boundary-condition values (reference node order)
+  std::vector<T> rho(h.N), vel(2 * (size_t)h.N);
+  const T *mr = static_cast<const T *>(m.node_rho);
+  const T *mv = static_cast<const T *>(m.node_vel);
+  for (int i = 0; i < h.N; ++i) {
+    rho[i] = mr ? mr[i] : T(0);
+    vel[i] = mv ? mv[2 * (size_t)i] : T(0);
+    vel[(size_t)h.N + i] = mv ? mv[2 * (size_t)i + 1] : T(0);
+  }
+  LHIP_TRY(hipMemcpy(h.node_rho, rho.data(), rho.size() * es, hipMemcpyHostToDevice));
+  LHIP_TRY(hipMemcpy(h.node_vel, vel.data(), vel.size() * es, hipMemcpyHostToDevice));
+  return init_uniform<T, QN>(h);
+}
+
+// derived fields of the pdf at the start of the last step
+template <typename T, int QN>
+static int observe(Handle &h) {
+  if (h.observed || !h.has_prev) return FVDBM_OK;
+  int rc = FVDBM_OK;
+  if (!h.obs_rho) rc = h.alloc(&h.obs_rho, (size_t)h.Cp * sizeof(T));
+  if (!rc && !h.obs_vel) rc = h.alloc(&h.obs_vel, (size_t)2 * h.Cp * sizeof(T));
+  if (!rc && !h.obs_feq) rc = h.alloc(&h.obs_feq, (size_t)QN * h.Cp * sizeof(T));
+  if (!rc && !h.obs_flux) rc = h.alloc(&h.obs_flux, (size_t)QN * h.F * sizeof(T));
+  if (rc) return rc;
+  const Args<T> a = make_args<T>(h, h.cur ^ 1);
+  const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+  hipLaunchKernelGGL((k_lat_moments<T, QN>), grid1(h.C), dim3(256), 0, h.stream, a, k,
+                     static_cast<T *>(h.obs_rho), static_cast<T *>(h.obs_vel),
+                     static_cast<T *>(h.obs_feq));
+  FaceArrays fa;
+  fa.c0 = static_cast<const int32_t *>(h.fc0);
+  fa.c1 = static_cast<const int32_t *>(h.fc1);
+  fa.n0 = static_cast<const int32_t *>(h.fn0);
+  fa.n1 = static_cast<const int32_t *>(h.fn1);
+  fa.d0 = h.fd0;
+  fa.d1 = h.fd1;
+  fa.nx = h.fnx;
+  fa.ny = h.fny;
+  fa.L = h.fL;
+  fa.F = h.F;
+  fa.last_cell = h.last_cell_new;
+  if (h.scheme == FVDBM_UPWIND)
+    hipLaunchKernelGGL((k_lat_flux<T, QN, 0>), grid1(h.F), dim3(256), 0, h.stream, a, k, fa,
+                       static_cast<T *>(h.obs_flux));
+  else
+    hipLaunchKernelGGL((k_lat_flux<T, QN, 1>), grid1(h.F), dim3(256), 0, h.stream, a, k, fa,
+                       static_cast<T *>(h.obs_flux));
+  LHIP_TRY(hipGetLastError());
+  LHIP_TRY(hipStreamSynchronize(h.stream));
+  h.observed = true;
+  return FVDBM_OK;
+}
+
+template <typename T, int QN>
+static int get_typed(Handle &h, int field, void *dst) {
+  T *out = static_cast<T *>(dst);
+  const size_t es = sizeof(T);
+  LHIP_TRY(hipStreamSynchronize(h.stream));
+  auto cells = [&](const void *dev, int comps) -> int {
+    std::vector<T> tmp((size_t)comps * h.Cp);
+    LHIP_TRY(hipMemcpy(tmp.data(), dev, tmp.size() * es, hipMemcpyDeviceToHost));
+    for (int i = 0; i < h.Cp; ++i) {
+      const int o = h.new_to_old[i];
+      if (o < 0) continue;
+      for (int q = 0; q < comps; ++q) out[(size_t)o * comps + q] = tmp[(size_t)q * h.Cp + i];
+    }
+    return FVDBM_OK;
+  };
+  auto nodes = [&](const void *dev, int comps) -> int {
+    std::vector<T> tmp((size_t)comps * h.N);
+    LHIP_TRY(hipMemcpy(tmp.data(), dev, tmp.size() * es, hipMemcpyDeviceToHost));
+    for (int i = 0; i < h.N; ++i)
+      for (int q = 0; q < comps; ++q) out[(size_t)i * comps + q] = tmp[(size_t)q * h.N + i];
+    return FVDBM_OK;
+  };
+  switch (field) {
+    case FVDBM_CELL_PDF:
+      return cells(h.f[h.cur], QN);
+    case FVDBM_CELL_RHO:
+    case FVDBM_CELL_VEL:
+    case FVDBM_CELL_PDF_EQ: {
+      if (!h.has_prev) {  // Cells.__init__: zeros until the first step
+        const size_t n = (size_t)h.C * (field == FVDBM_CELL_RHO ? 1 : field == FVDBM_CELL_VEL ? 2 : QN);
+        std::fill(out, out + n, T(0));
+        return FVDBM_OK;
+      }
+      int rc = observe<T, QN>(h);
+      if (rc) return rc;
+      if (field == FVDBM_CELL_RHO) return cells(h.obs_rho, 1);
+      if (field == FVDBM_CELL_VEL) return cells(h.obs_vel, 2);
+      return cells(h.obs_feq, QN);
+    }
+    case FVDBM_FACE_FLUX: {
+      if (!h.has_prev) {  // Faces.__init__: W rows
+        const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+        for (int j = 0; j < h.F; ++j)
+          for (int q = 0; q < QN; ++q) out[(size_t)j * QN + q] = k.W[q];
+        return FVDBM_OK;
+      }
+      int rc = observe<T, QN>(h);
+      if (rc) return rc;
+      std::vector<T> tmp((size_t)QN * h.F);
+      LHIP_TRY(hipMemcpy(tmp.data(), h.obs_flux, tmp.size() * es, hipMemcpyDeviceToHost));
+      for (int j = 0; j < h.F; ++j)
+        for (int q = 0; q < QN; ++q) out[(size_t)j * QN + q] = tmp[(size_t)q * h.F + j];
+      return FVDBM_OK;
+    }
+    case FVDBM_NODE_PDF:
+      return nodes(h.node_pdf, QN);
+    case FVDBM_NODE_RHO:
+      LHIP_TRY(hipMemcpy(out, h.node_rho, (size_t)h.N * es, hipMemcpyDeviceToHost));
+      return FVDBM_OK;
+    case FVDBM_NODE_VEL:
+      return nodes(h.node_vel, 2);
+    default:
+      return fail(FVDBM_ERR_INVALID, "unknown field");
+  }
+}
+
+template <typename T, int QN>
+static int set_typed(Handle &h, int field, const void *src) {
+  const T *in = static_cast<const T *>(src);
+  const size_t es = sizeof(T);
+  LHIP_TRY(hipStreamSynchronize(h.stream));
+  if (field == FVDBM_CELL_PDF) {
+    // the current buffer only: the other keeps the start of the last step
+    std::vector<T> tmp((size_t)QN * h.Cp, T(0));
+    for (int i = 0; i < h.Cp; ++i) {
+      const int o = h.new_to_old[i];
+      if (o < 0) continue;
+      for (int q = 0; q < QN; ++q) tmp[(size_t)q * h.Cp + i] = in[(size_t)o * QN + q];
+    }
+    LHIP_TRY(hipMemcpy(h.f[h.cur], tmp.data(), tmp.size() * es, hipMemcpyHostToDevice));
+    return FVDBM_OK;
+  }
+  if (field == FVDBM_NODE_PDF || field == FVDBM_NODE_VEL) {
+    const int comps = field == FVDBM_NODE_PDF ? QN : 2;
+    std::vector<T> tmp((size_t)comps * h.N);
+    for (int i = 0; i < h.N; ++i)
+      for (int q = 0; q < comps; ++q) tmp[(size_t)q * h.N + i] = in[(size_t)i * comps + q];
+    LHIP_TRY(hipMemcpy(field == FVDBM_NODE_PDF ? h.node_pdf : h.node_vel, tmp.data(),
+                       tmp.size() * es, hipMemcpyHostToDevice));
+    if (field == FVDBM_NODE_VEL && h.nan) {  // velocity-node values feed the step
+      std::vector<T> ux(h.nan), uy(h.nan);
+      for (int a = 0; a < h.nan; ++a) {
+        ux[a] = in[2 * (size_t)h.an_gid_host[a]];
+        uy[a] = in[2 * (size_t)h.an_gid_host[a] + 1];
+      }
+      LHIP_TRY(hipMemcpy(h.an_ux, ux.data(), ux.size() * es, hipMemcpyHostToDevice));
+      LHIP_TRY(hipMemcpy(h.an_uy, uy.data(), uy.size() * es, hipMemcpyHostToDevice));
+    }
+    h.observed = false;
+    return FVDBM_OK;
+  }
+  if (field == FVDBM_NODE_RHO) {
+    LHIP_TRY(hipMemcpy(h.node_rho, in, (size_t)h.N * es, hipMemcpyHostToDevice));
+    if (h.nan) {  // density-node values feed the step
+      std::vector<T> r(h.nan);
+      for (int a = 0; a < h.nan; ++a) r[a] = in[h.an_gid_host[a]];
+      LHIP_TRY(hipMemcpy(h.an_rho, r.data(), r.size() * es, hipMemcpyHostToDevice));
+    }
+    h.observed = false;
+    return FVDBM_OK;
+  }
+  return fail(FVDBM_ERR_INVALID, "field cannot be written (derived from the pdf)");
+}
+
+}  // namespace fvdbm_lattice
+
+using fvdbm_lattice::Handle;
+
+extern "C" {
+
+int fvdbm_lattice_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, int32_t quivers,
+                         void **out) {
+  using namespace fvdbm_lattice;
+  if (!mesh || !p || !out) return fail(FVDBM_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (quivers != 9 && quivers != 13) return fail(FVDBM_ERR_INVALID, "quivers must be 9 or 13");
+  if (p->precision != FVDBM_FP32 && p->precision != FVDBM_FP64)
+    return fail(FVDBM_ERR_INVALID, "precision must be FVDBM_FP32 or FVDBM_FP64");
+  if (p->scheme != FVDBM_UPWIND && p->scheme != FVDBM_LAX_WENDROFF)
+    return fail(FVDBM_ERR_INVALID, "Unknown flux scheme");
+  if (!(p->tau > 0.0) || !std::isfinite(p->delta_t))
+    return fail(FVDBM_ERR_INVALID, "tau must be > 0 and delta_t finite");
+  const int block = p->block ? p->block : 256;
+  if (block < 64 || block > 1024 || block % 64)
+    return fail(FVDBM_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+  int gs = p->graph_steps == 0 ? 32 : p->graph_steps;
+  if (gs > 0 && (gs % 2)) ++gs;  // even, so a replay preserves buffer parity
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(FVDBM_ERR_HIP, "no HIP device available");
+  std::unique_ptr<Handle> h(new Handle());
+  h->q = quivers;
+  h->precision = p->precision;
+  h->scheme = p->scheme;
+  h->block = block;
+  h->graph_steps = gs;
+  h->tau = p->tau;
+  h->dt = p->delta_t;
+  LHIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  LHIP_TRY(hipEventCreate(&h->ev0));
+  LHIP_TRY(hipEventCreate(&h->ev1));
+  const int reorder = p->reorder;
+  int rc = dispatch(*h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return create_typed<T, decltype(qn)::value>(*mesh, *h, reorder);
+  });
+  if (rc) return rc;
+  *out = h.release();
+  return FVDBM_OK;
+}
+
+int fvdbm_lattice_destroy(void *handle) {
+  delete static_cast<Handle *>(handle);
+  return FVDBM_OK;
+}
+
+int fvdbm_lattice_init_uniform(void *handle) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  return dispatch(*h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return init_uniform<T, decltype(qn)::value>(*h);
+  });
+}
+
+int fvdbm_lattice_step(void *handle, int64_t nsteps) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
+  return run_steps(*h, nsteps);
+}
+
+int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
+  if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
+  if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
+    int rc = ensure_graph(*h, h->cur);  // outside the timed region
+    if (rc) return rc;
+  }
+  LHIP_TRY(hipEventRecord(h->ev0, h->stream));
+  int rc = run_steps(*h, nsteps);
+  if (rc) return rc;
+  LHIP_TRY(hipEventRecord(h->ev1, h->stream));
+  LHIP_TRY(hipEventSynchronize(h->ev1));
+  float t = 0.f;
+  LHIP_TRY(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  *ms = t;
+  return FVDBM_OK;
+}
+
+int fvdbm_lattice_synchronize(void *handle) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  LHIP_TRY(hipStreamSynchronize(h->stream));
+  return FVDBM_OK;
+}
+
+int fvdbm_lattice_get_field(void *handle, int32_t field, void *dst) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !dst) return fail(FVDBM_ERR_INVALID, "null argument");
+  return dispatch(*h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return get_typed<T, decltype(qn)::value>(*h, field, dst);
+  });
+}
+
+int fvdbm_lattice_set_field(void *handle, int32_t field, const void *src) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !src) return fail(FVDBM_ERR_INVALID, "null argument");
+  return dispatch(*h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return set_typed<T, decltype(qn)::value>(*h, field, src);
+  });
+}
+
+int fvdbm_lattice_info_get(void *handle, fvdbm_info *o) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !o) return fail(FVDBM_ERR_INVALID, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->n_cells = h->C;
+  o->n_faces = h->F;
+  o->n_nodes = h->N;
+  o->k_max = h->K;
+  o->n_cells_padded = h->Cp;
+  o->n_boundary_cells = h->nbc;
+  o->n_boundary_faces = h->nbf;
+  o->n_active_nodes = h->nan;
+  o->bandwidth = h->bandwidth;
+  o->precision = h->precision;
+  o->scheme = h->scheme;
+  o->variant = 2 | 4 | 8;  // node kernel, no staging, one step per launch
+  o->block = h->block;
+  o->graph_steps = h->graph_steps;
+  o->steps_done = h->steps;
+  o->device_bytes = h->device_bytes;
+  o->n_cases = 1;
+  return FVDBM_OK;
+}
+
+int fvdbm_lattice_get_permutation(void *handle, int32_t *new_to_old) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h || !new_to_old) return fail(FVDBM_ERR_INVALID, "null argument");
+  for (int i = 0; i < h->C; ++i) new_to_old[i] = h->new_to_old[i];
+  return FVDBM_OK;
+}
+
+}  // extern "C"

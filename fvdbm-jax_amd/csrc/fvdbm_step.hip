@@ -29,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "errors.h"
 #include "fvdbm.h"
 #include "layout.h"
 
@@ -43,6 +44,8 @@ static int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
 
 #define HIP_TRY(expr)                                                        \
   do {                                                                       \

@@ -15,7 +15,11 @@ Differences from the reference, by design:
     captured hipGraphs; ``step_residual`` and ``run_to_steady`` add the
     convergence residual the reference lacks;
   * ``precision="fp64"`` runs the same kernels in double precision (the
-    FP64 twin used to state the FP32 error).
+    FP64 twin used to state the FP32 error);
+  * an Environment on ``D2Q13`` dynamics (or any, with
+    ``lattice_kernel=True``) runs on the library's any-lattice handle
+    (fvdbm_lattice_*: plain kernels, one step per launch); the residual,
+    warm-start and field-interpolation methods need the tuned D2Q9 handle.
 """
 from __future__ import annotations
 
@@ -26,16 +30,22 @@ import numpy as np
 from . import _lib
 from .containers import Cells, Faces, Nodes
 
+_Q = None  # the lattice's pdf width
 _FIELDS = {
-    ("cells", "pdf"): (_lib.CELL_PDF, 9),
+    ("cells", "pdf"): (_lib.CELL_PDF, _Q),
     ("cells", "rho"): (_lib.CELL_RHO, 1),
     ("cells", "vel"): (_lib.CELL_VEL, 2),
-    ("cells", "pdf_eq"): (_lib.CELL_PDF_EQ, 9),
-    ("faces", "pdf"): (_lib.FACE_FLUX, 9),
-    ("nodes", "pdf"): (_lib.NODE_PDF, 9),
+    ("cells", "pdf_eq"): (_lib.CELL_PDF_EQ, _Q),
+    ("faces", "pdf"): (_lib.FACE_FLUX, _Q),
+    ("nodes", "pdf"): (_lib.NODE_PDF, _Q),
     ("nodes", "rho"): (_lib.NODE_RHO, 1),
     ("nodes", "vel"): (_lib.NODE_VEL, 2),
 }
+# entry points of the two handle kinds
+_TUNED_API = dict(destroy="fvdbm_destroy", set="fvdbm_set_field", get="fvdbm_get_field",
+                  step="fvdbm_step", timed="fvdbm_step_timed", sync="fvdbm_synchronize",
+                  info="fvdbm_info_get", perm="fvdbm_get_permutation")
+_LATTICE_API = {k: v.replace("fvdbm_", "fvdbm_lattice_", 1) for k, v in _TUNED_API.items()}
 _SETTABLE = {("cells", "pdf"), ("nodes", "pdf"), ("nodes", "rho"), ("nodes", "vel")}
 
 
@@ -50,7 +60,7 @@ class Environment(_lib.FieldsMixin):
     dynamics = None  # class attribute used by create(), as in the reference
 
     def __init__(self, cells: Cells, faces: Faces, nodes: Nodes, precision="fp32",
-                 reorder=None, variant=0, block=0, graph_steps=0):
+                 reorder=None, variant=0, block=0, graph_steps=0, lattice_kernel=False):
         self.cells = cells
         self.faces = faces
         self.nodes = nodes
@@ -58,8 +68,10 @@ class Environment(_lib.FieldsMixin):
             raise ValueError("precision must be 'fp32' or 'fp64'")
         self.precision = precision
         self.options = dict(reorder=reorder, variant=variant, block=block,
-                            graph_steps=graph_steps)
+                            graph_steps=graph_steps, lattice_kernel=bool(lattice_kernel))
         self._h = None
+        self._api = _TUNED_API
+        self._lattice = False
 
     @classmethod
     def create(cls, num_cells, num_faces, num_nodes):
@@ -126,7 +138,15 @@ class Environment(_lib.FieldsMixin):
                         int(reorder), int(self.options["variant"]),
                         int(self.options["block"]), int(self.options["graph_steps"]))
         h = ct.c_void_p()
-        _lib.check(_lib.lib().fvdbm_create(ct.byref(desc), ct.byref(p), ct.byref(h)), "fvdbm_create")
+        quivers = int(dyn.NUM_QUIVERS)
+        self._lattice = quivers != 9 or self.options["lattice_kernel"]
+        if self._lattice:
+            _lib.check(_lib.lib().fvdbm_lattice_create(ct.byref(desc), ct.byref(p), quivers, ct.byref(h)),
+                       "fvdbm_lattice_create")
+            self._api = _LATTICE_API
+        else:
+            _lib.check(_lib.lib().fvdbm_create(ct.byref(desc), ct.byref(p), ct.byref(h)), "fvdbm_create")
+            self._api = _TUNED_API
         self._h = h
         self._arrays = arrays  # keep host copies alive for the handle's lifetime
         # initial state: fvdbm_create set every pdf to W in the working
@@ -135,13 +155,13 @@ class Environment(_lib.FieldsMixin):
         for field, obj in ((_lib.CELL_PDF, self.cells), (_lib.NODE_PDF, self.nodes)):
             host = np.asarray(obj.__dict__["_host_pdf"])
             if not (host.dtype == np.float32 and np.array_equal(host, np.broadcast_to(W32, host.shape))):
-                self._upload(field, host, 9)
+                self._upload(field, host, quivers)
         for obj in (self.cells, self.faces, self.nodes):
             obj.__dict__["_env"] = self
 
     def close(self):
         if getattr(self, "_h", None):
-            _lib.lib().fvdbm_destroy(self._h)
+            getattr(_lib.lib(), self._api["destroy"])(self._h)
             self._h = None
             for obj in (self.cells, self.faces, self.nodes):
                 obj.__dict__["_env"] = None
@@ -155,34 +175,50 @@ class Environment(_lib.FieldsMixin):
     def _counts(self):
         return self.cells.size, self.nodes.size
 
-    def _handle(self):
+    def _core(self):
         if not self._h:
             raise RuntimeError("Environment.init() has not been called")
         return self._h
 
+    def _handle(self):
+        """The tuned D2Q9 handle (residual, warm start and field interpolation)."""
+        h = self._core()
+        if self._lattice:
+            raise RuntimeError("this operation needs the tuned D2Q9 handle; this Environment runs "
+                               "on the any-lattice handle")
+        return h
+
+    def _call(self, op, *args):
+        name = self._api[op]
+        _lib.check(getattr(_lib.lib(), name)(self._core(), *args), name)
+
+    def _comps(self, field_comps):
+        return int(self.cells.dynamics.NUM_QUIVERS) if field_comps is _Q else field_comps
+
     def _upload(self, field, value, comps):
         arr = np.ascontiguousarray(np.asarray(value, dtype=self.dtype).reshape(-1, comps))
-        _lib.check(_lib.lib().fvdbm_set_field(self._handle(), field, _lib.ptr(arr)), "fvdbm_set_field")
+        self._call("set", field, _lib.ptr(arr))
 
     # container attribute plumbing ---------------------------------------
     def _read(self, kind, name):
         field, comps = _FIELDS[(kind, name)]
+        comps = self._comps(comps)
         n = {"cells": self.cells, "faces": self.faces, "nodes": self.nodes}[kind].size
         out = np.empty((n, comps), dtype=self.dtype)
-        _lib.check(_lib.lib().fvdbm_get_field(self._handle(), field, _lib.ptr(out)), "fvdbm_get_field")
+        self._call("get", field, _lib.ptr(out))
         return out
 
     def _write(self, kind, name, value):
         if (kind, name) not in _SETTABLE:
             raise ValueError(f"{kind}.{name} is derived from the pdf and cannot be set")
         field, comps = _FIELDS[(kind, name)]
-        self._upload(field, value, comps)
+        self._upload(field, value, self._comps(comps))
 
     # stepping --------------------------------------------------------------
     def step(self, nsteps=1):
         """One FVDBM step (or ``nsteps``): macros -> equilibrium -> node pdfs
         -> face fluxes -> cell update (src/environment.py:55-65). Returns self."""
-        _lib.check(_lib.lib().fvdbm_step(self._handle(), int(nsteps)), "fvdbm_step")
+        self._call("step", int(nsteps))
         return self
 
     def step_residual(self, nsteps=1):
@@ -210,22 +246,20 @@ class Environment(_lib.FieldsMixin):
     def time_steps(self, nsteps):
         """Run ``nsteps`` between HIP events; returns elapsed milliseconds."""
         ms = ct.c_double()
-        _lib.check(_lib.lib().fvdbm_step_timed(self._handle(), int(nsteps), ct.byref(ms)),
-                   "fvdbm_step_timed")
+        self._call("timed", int(nsteps), ct.byref(ms))
         return float(ms.value)
 
     def synchronize(self):
-        _lib.check(_lib.lib().fvdbm_synchronize(self._handle()), "fvdbm_synchronize")
+        self._call("sync")
 
     def info(self):
         inf = _lib.Info()
-        _lib.check(_lib.lib().fvdbm_info_get(self._handle(), ct.byref(inf)), "fvdbm_info_get")
+        self._call("info", ct.byref(inf))
         return {k: getattr(inf, k) for k, _ in inf._fields_}
 
     def permutation(self):
         out = np.empty(self.cells.size, np.int32)
-        _lib.check(_lib.lib().fvdbm_get_permutation(self._handle(), _lib.ptr(out)),
-                   "fvdbm_get_permutation")
+        self._call("perm", _lib.ptr(out))
         return out
 
     def __repr__(self):

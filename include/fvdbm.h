@@ -160,6 +160,29 @@ int fvdbm_step_residual(void *handle, int64_t nsteps, double *residual);
 int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms);
 int fvdbm_synchronize(void *handle);
 
+/* ------------------------------------------------------------------------
+ * Any-lattice handle: D2Q13 (src/dynamics.py:77-102; quivers = 13) and D2Q9
+ * (quivers = 9, the same plain kernels, a cross-check of the tuned path).
+ * Replaces Environment.init/step for an Environment built on D2Q13
+ * (src/environment.py:38-65). Mesh descriptor and params as for
+ * fvdbm_create (params.variant is ignored: one step per launch, boundary
+ * nodes in their own kernel, no LDS staging); fields as for
+ * fvdbm_get_field / fvdbm_set_field with Q = quivers pdf components.
+ * Errors through fvdbm_last_error(). No batched handles, residual or
+ * warm-start entry points for this handle.
+ * ------------------------------------------------------------------------ */
+int fvdbm_lattice_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *params,
+                         int32_t quivers, void **handle);
+int fvdbm_lattice_destroy(void *handle);
+int fvdbm_lattice_init_uniform(void *handle);
+int fvdbm_lattice_step(void *handle, int64_t nsteps);
+int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms);
+int fvdbm_lattice_synchronize(void *handle);
+int fvdbm_lattice_get_field(void *handle, int32_t field, void *dst);
+int fvdbm_lattice_set_field(void *handle, int32_t field, const void *src);
+int fvdbm_lattice_info_get(void *handle, fvdbm_info *out);
+int fvdbm_lattice_get_permutation(void *handle, int32_t *new_to_old);
+
 /* Measurement aid (no reference counterpart): copies `bytes` between two
  * fresh device buffers `reps` times with a plain 16-byte-per-lane kernel
  * (k_copy_probe) and returns milliseconds per copy. Used to scale the

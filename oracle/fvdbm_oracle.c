@@ -28,7 +28,12 @@
  * mistake tests under tests/.
  *
  * Layouts are the reference container layouts, row-major:
- *   cell pdf [C][9], face flux [F][9], node pdf [N][9], vel [.][2].
+ *   cell pdf [C][Q], face flux [F][Q], node pdf [N][Q], vel [.][2].
+ *
+ * Two lattices: D2Q9 (oracle_step_f32/_f64, Q = 9) and D2Q13
+ * (oracle_step_q13_f32/_f64; src/dynamics.py:77-102: the D2Q9 velocities
+ * plus (+-2,0), (0,+-2), W = 3/8, 1/12, 1/16, 1/96, C = 1/sqrt(2) and the
+ * cubic equilibrium terms). The same step body is compiled for each.
  */
 #include <math.h>
 #include <stdint.h>
@@ -95,6 +100,8 @@ typedef struct {
 
 struct consts_f32 { float W[Q], C, C2, C4x2, C2x2; };
 struct consts_f64 { double W[Q], C, C2, C4x2, C2x2; };
+struct consts_q13_f32 { float W[13], C, C2, C4x2, C2x2, C6x2; };
+struct consts_q13_f64 { double W[13], C, C2, C4x2, C2x2, C6x2; };
 
 /* W = jnp.array([4/9, 1/9 x4, 1/36 x4]) rounds the Python doubles;
  * C = 1/jnp.sqrt(3) (src/dynamics.py:63-64). */
@@ -130,6 +137,74 @@ static void init_consts_f64(struct consts_f64 *k) {
 #include "oracle_step.inc"
 #undef REAL
 #undef SFX
+
+/* D2Q13 (src/dynamics.py:77-102). W = jnp.array([3/8, 1/12 x4, 1/16 x4,
+ * 1/96 x4]) rounds the Python doubles; C = 1/jnp.sqrt(2); powers of C as
+ * lax.integer_pow computes them (by squaring): C^2 = C*C, C^4 = C^2*C^2,
+ * C^6 = C^2*C^4. */
+static const int KSI13[13][2] = {{0, 0},  {1, 0},  {0, 1},   {-1, 0}, {0, -1},
+                                 {1, 1},  {-1, 1}, {-1, -1}, {1, -1}, {2, 0},
+                                 {0, 2},  {-2, 0}, {0, -2}};
+static const double W13[13] = {3.0 / 8,  1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12,
+                               1.0 / 16, 1.0 / 16, 1.0 / 16, 1.0 / 16, 1.0 / 96,
+                               1.0 / 96, 1.0 / 96, 1.0 / 96};
+static void init_consts_q13_f32(struct consts_q13_f32 *k) {
+  for (int q = 0; q < 13; ++q) k->W[q] = (float)W13[q];
+  k->C = 1.0f / sqrtf(2.0f);
+  k->C2 = k->C * k->C;
+  float c4 = k->C2 * k->C2;
+  float c6 = k->C2 * c4;
+  k->C4x2 = 2.0f * c4;
+  k->C2x2 = 2.0f * k->C2;
+  k->C6x2 = 2.0f * c6;
+}
+static void init_consts_q13_f64(struct consts_q13_f64 *k) {
+  for (int q = 0; q < 13; ++q) k->W[q] = W13[q];
+  k->C = 1.0 / sqrt(2.0);
+  k->C2 = k->C * k->C;
+  double c4 = k->C2 * k->C2;
+  double c6 = k->C2 * c4;
+  k->C4x2 = 2.0 * c4;
+  k->C2x2 = 2.0 * k->C2;
+  k->C6x2 = 2.0 * c6;
+}
+
+#undef Q
+#define Q 13
+#define KSI KSI13
+#define REAL float
+#define SFX q13_f32
+#include "oracle_step.inc"
+#undef REAL
+#undef SFX
+#define REAL double
+#define SFX q13_f64
+#include "oracle_step.inc"
+#undef REAL
+#undef SFX
+
+int oracle_calc_eq_q13_f32(int n, const float *rho, const float *vel, float *out) {
+  struct consts_q13_f32 k;
+  init_consts_q13_f32(&k);
+  for (int i = 0; i < n; ++i)
+    calc_eq_q13_f32(&k, rho[i], vel[2 * i], vel[2 * i + 1], out + (size_t)i * 13, 0);
+  return 0;
+}
+int oracle_calc_eq_q13_f64(int n, const double *rho, const double *vel, double *out) {
+  struct consts_q13_f64 k;
+  init_consts_q13_f64(&k);
+  for (int i = 0; i < n; ++i)
+    calc_eq_q13_f64(&k, rho[i], vel[2 * i], vel[2 * i + 1], out + (size_t)i * 13, 0);
+  return 0;
+}
+int oracle_calc_macro_q13_f32(int n, const float *f, float *rho, float *vel) {
+  for (int i = 0; i < n; ++i)
+    calc_macro_q13_f32(f + (size_t)i * 13, rho + i, vel + 2 * i, vel + 2 * i + 1, 0);
+  return 0;
+}
+#undef KSI
+#undef Q
+#define Q 9
 
 /* Equilibrium of (rho, u) for a batch of points, used by tests for the
  * closed-form moment checks and for warm-start initialisation checks. */

@@ -28,6 +28,16 @@ MUTATIONS = {
 KSI = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1],
                 [1, 1], [-1, 1], [-1, -1], [1, -1]], dtype=np.int32)
 W64 = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
+# D2Q13, src/dynamics.py:80-96
+KSI13 = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1],
+                  [2, 0], [0, 2], [-2, 0], [0, -2]], dtype=np.int32)
+W13_64 = np.array([3 / 8] + [1 / 12] * 4 + [1 / 16] * 4 + [1 / 96] * 4)
+
+
+def _sfx(precision, quivers):
+    if quivers not in (9, 13):
+        raise ValueError("quivers must be 9 (D2Q9) or 13 (D2Q13)")
+    return ("q13_" if quivers == 13 else "") + precision
 
 
 class _Mesh(ct.Structure):
@@ -60,13 +70,15 @@ def lib():
         if not os.path.exists(_SO):
             build()
         _lib = ct.CDLL(_SO)
-        for name in ("oracle_step_f32", "oracle_step_f64"):
+        for name in ("oracle_step_f32", "oracle_step_f64", "oracle_step_q13_f32",
+                     "oracle_step_q13_f64"):
             fn = getattr(_lib, name)
             fn.restype = ct.c_int
             fn.argtypes = [ct.POINTER(_Mesh), ct.c_double, ct.c_double,
                            ct.POINTER(_State), ct.c_int, ct.c_int, ct.c_int]
-        for name in ("oracle_calc_eq_f32", "oracle_calc_eq_f64",
-                     "oracle_calc_macro_f32"):
+        for name in ("oracle_calc_eq_f32", "oracle_calc_eq_f64", "oracle_calc_macro_f32",
+                     "oracle_calc_eq_q13_f32", "oracle_calc_eq_q13_f64",
+                     "oracle_calc_macro_q13_f32"):
             fn = getattr(_lib, name)
             fn.restype = ct.c_int
             fn.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
@@ -86,13 +98,14 @@ def _ptr(a):
     return a.ctypes.data_as(ct.c_void_p)
 
 
-def uniform_w(n, precision="f32"):
+def uniform_w(n, precision="f32", quivers=9):
     """Container.__init__ pdf: calc_eq(1, [0,0]) repeated (src/containers.py:29)."""
     ft = np.float32 if precision == "f32" else np.float64
-    return np.repeat(W64.astype(ft)[None, :], n, axis=0)
+    w = W13_64 if quivers == 13 else W64
+    return np.repeat(w.astype(ft)[None, :], n, axis=0)
 
 
-def initial_state(env, node_type, node_rho=None, node_vel=None, precision="f32"):
+def initial_state(env, node_type, node_rho=None, node_vel=None, precision="f32", quivers=9):
     """State right after ``Environment.init()``: every pdf is W, moments and
     pdf_eq are zero, node rho/vel are the boundary-condition values."""
     ft = np.float32 if precision == "f32" else np.float64
@@ -100,19 +113,21 @@ def initial_state(env, node_type, node_rho=None, node_vel=None, precision="f32")
     F = env["face_nodes"].shape[0]
     N = env["node_cells"].shape[0]
     return dict(
-        cell_pdf=uniform_w(C, precision), cell_rho=np.zeros(C, ft),
-        cell_vel=np.zeros((C, 2), ft), cell_feq=np.zeros((C, 9), ft),
-        face_pdf=uniform_w(F, precision), node_pdf=uniform_w(N, precision),
+        cell_pdf=uniform_w(C, precision, quivers), cell_rho=np.zeros(C, ft),
+        cell_vel=np.zeros((C, 2), ft), cell_feq=np.zeros((C, quivers), ft),
+        face_pdf=uniform_w(F, precision, quivers), node_pdf=uniform_w(N, precision, quivers),
         node_rho=np.zeros(N, ft) if node_rho is None else np.asarray(node_rho, ft).reshape(N).copy(),
         node_vel=np.zeros((N, 2), ft) if node_vel is None else np.asarray(node_vel, ft).reshape(N, 2).copy(),
         node_type=np.asarray(node_type, np.int32).reshape(N).copy(),
     )
 
 
-def run(env, state, tau, dt, nsteps, precision="f32", mutation="none", scheme="upwind"):
+def run(env, state, tau, dt, nsteps, precision="f32", mutation="none", scheme="upwind",
+        quivers=9):
     """Advance ``state`` (dict from ``initial_state``) by ``nsteps`` in place
     and return it. ``env`` holds the to_env arrays in the working precision;
-    ``scheme`` is "upwind" or "lax_wendroff"."""
+    ``scheme`` is "upwind" or "lax_wendroff"; ``quivers`` 9 (D2Q9) or 13
+    (D2Q13), matching the state's pdf width."""
     ft = np.float32 if precision == "f32" else np.float64
     arrs = {}
     for k in ("cell_faces", "cell_signs", "face_cells", "face_nodes", "node_cells"):
@@ -131,7 +146,9 @@ def run(env, state, tau, dt, nsteps, precision="f32", mutation="none", scheme="u
     s = _State(*[_ptr(state[k]) for k in (
         "cell_pdf", "cell_rho", "cell_vel", "cell_feq", "face_pdf",
         "node_pdf", "node_rho", "node_vel")])
-    fn = lib().oracle_step_f32 if precision == "f32" else lib().oracle_step_f64
+    if state["cell_pdf"].shape[1] != quivers:
+        raise ValueError("state pdf width does not match quivers")
+    fn = getattr(lib(), "oracle_step_" + _sfx(precision, quivers))
     rc = fn(ct.byref(m), float(tau), float(dt), ct.byref(s), int(nsteps),
             {"upwind": 0, "lax_wendroff": 1}[scheme], MUTATIONS[mutation])
     if rc != 0:
@@ -139,21 +156,22 @@ def run(env, state, tau, dt, nsteps, precision="f32", mutation="none", scheme="u
     return state
 
 
-def calc_eq(rho, vel, precision="f32"):
-    """D2Q9.calc_eq for arrays rho[n], vel[n,2] (src/dynamics.py:70-74)."""
+def calc_eq(rho, vel, precision="f32", quivers=9):
+    """D2Q9.calc_eq (src/dynamics.py:70-74) or D2Q13.calc_eq (:101-102) for
+    arrays rho[n], vel[n,2]."""
     ft = np.float32 if precision == "f32" else np.float64
     rho = np.ascontiguousarray(rho, ft).reshape(-1)
     vel = np.ascontiguousarray(vel, ft).reshape(-1, 2)
-    out = np.empty((rho.shape[0], 9), ft)
-    fn = lib().oracle_calc_eq_f32 if precision == "f32" else lib().oracle_calc_eq_f64
+    out = np.empty((rho.shape[0], quivers), ft)
+    fn = getattr(lib(), "oracle_calc_eq_" + _sfx(precision, quivers))
     fn(rho.shape[0], _ptr(rho), _ptr(vel), _ptr(out))
     return out
 
 
-def calc_macro(f):
-    """Dynamics.calc_macro for f[n,9] in FP32 (src/dynamics.py:35-47)."""
-    f = np.ascontiguousarray(f, np.float32).reshape(-1, 9)
+def calc_macro(f, quivers=9):
+    """Dynamics.calc_macro for f[n,Q] in FP32 (src/dynamics.py:35-47)."""
+    f = np.ascontiguousarray(f, np.float32).reshape(-1, quivers)
     rho = np.empty(f.shape[0], np.float32)
     vel = np.empty((f.shape[0], 2), np.float32)
-    lib().oracle_calc_macro_f32(f.shape[0], _ptr(f), _ptr(rho), _ptr(vel))
+    getattr(lib(), "oracle_calc_macro_" + _sfx("f32", quivers))(f.shape[0], _ptr(f), _ptr(rho), _ptr(vel))
     return rho, vel

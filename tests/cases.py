@@ -19,22 +19,25 @@ def build_mesher(raw):
     return m
 
 
-def ldc_case(n=32, tau=LDC["tau"], dt=LDC["dt"], u_lid=LDC["u_lid"], scheme="upwind"):
+def ldc_case(n=32, tau=LDC["tau"], dt=LDC["dt"], u_lid=LDC["u_lid"], scheme="upwind",
+             lattice=None):
     """The 32x32 lid-driven cavity: lid nodes u=(u_lid,0), all other boundary
-    nodes (lid corners included) u=0, interior nodes type 0."""
+    nodes (lid corners included) u=0, interior nodes type 0. ``lattice``:
+    the dynamics class (default D2Q9)."""
     m = build_mesher(meshgen.ldc_structured(n))
-    dyn = fv.D2Q9(tau=tau, delta_t=dt)
+    dyn = (lattice or fv.D2Q9)(tau=tau, delta_t=dt)
     cells, faces, nodes = m.to_env(dyn, scheme, dim_multiplier=1)
     nodes = m.set_vel_node(nodes, meshgen.LID, np.array([u_lid, 0.0]))
     nodes = m.set_vel_node(nodes, meshgen.WALL, np.array([0.0, 0.0]))
     return m, dyn, cells, faces, nodes
 
 
-def channel_case(raw, tau, dt, u_max, dim_multiplier, rho_out=1.0, scheme="upwind"):
+def channel_case(raw, tau, dt, u_max, dim_multiplier, rho_out=1.0, scheme="upwind",
+                 lattice=None):
     """Velocity inlet (parabolic, per node), density outlet, no-slip walls and
     obstacles. Walls are set last so channel corners are walls."""
     m = build_mesher(raw)
-    dyn = fv.D2Q9(tau=tau, delta_t=dt)
+    dyn = (lattice or fv.D2Q9)(tau=tau, delta_t=dt)
     cells, faces, nodes = m.to_env(dyn, scheme, dim_multiplier=dim_multiplier)
     H = raw.meta.get("height", 0.41)
     nodes = m.set_vel_profile(nodes, meshgen.INLET,
@@ -61,12 +64,12 @@ def oracle_env(cells, faces, nodes, precision="f32"):
                 node_dists=nodes.cell_dists_f64)
 
 
-def oracle_state(env, nodes, precision="f32"):
+def oracle_state(env, nodes, precision="f32", quivers=9):
     from oracle import oracle
     if precision == "f32":
         return oracle.initial_state(env, nodes.type, nodes.__dict__["_host_rho"],
-                                    nodes.__dict__["_host_vel"], "f32")
-    return oracle.initial_state(env, nodes.type, nodes.rho_f64, nodes.vel_f64, "f64")
+                                    nodes.__dict__["_host_vel"], "f32", quivers)
+    return oracle.initial_state(env, nodes.type, nodes.rho_f64, nodes.vel_f64, "f64", quivers)
 
 
 def mean_area(mesher):
