@@ -51,6 +51,13 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_LUP_F32 = 108  # pdf 36 read + 36 write, 3 neighbour ids 12, 3 (L n) pairs 24
 BYTES_PER_LUP_F64 = 204
+
+
+def bytes_per_lup(quivers, precision):
+    """Algorithmic bytes of one cell update: pdf read and write (Q values
+    each), 3 neighbour ids, 3 (L, n) pairs."""
+    es = 4 if precision == "fp32" else 8
+    return 2 * quivers * es + 12 + 3 * 2 * es
 HBM_PEAK_GBS = 8000.0
 
 
@@ -58,7 +65,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_case(cells, seed, scheme="upwind", cache_dir=None):
+def build_case(cells, seed, scheme="upwind", cache_dir=None, lattice="d2q9"):
     fv = importlib.import_module("fvdbm-jax_amd")
     mg = fv.meshgen
     t0 = time.time()
@@ -80,10 +87,11 @@ def build_case(cells, seed, scheme="upwind", cache_dir=None):
                         - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
     mult = float(1.0 / area.mean())
     tau = 0.55
-    dyn = fv.D2Q9(tau=tau, delta_t=1.0)
+    dyn = (fv.D2Q13 if lattice == "d2q13" else fv.D2Q9)(tau=tau, delta_t=1.0)
     c, f, n = m.to_env(dyn, scheme, dim_multiplier=mult)
     L = np.asarray(f.L, np.float64).reshape(-1)
-    per_cell = L[m.cell_face_indices].sum(axis=1) * np.sqrt(2.0)
+    # explicit-Euler bound with the lattice's largest speed (sqrt 2, or 2 for D2Q13)
+    per_cell = L[m.cell_face_indices].sum(axis=1) * (2.0 if lattice == "d2q13" else np.sqrt(2.0))
     dyn.delta_t = float(np.float32(0.5 / (1.0 / tau + per_cell.max())))
     n = m.set_vel_profile(n, mg.INLET, lambda q: mg.parabolic_inlet(q[:, 1], 0.41, 0.1))
     n = m.set_rho_node(n, mg.OUTLET, 1.0)
@@ -92,6 +100,16 @@ def build_case(cells, seed, scheme="upwind", cache_dir=None):
     log(f"mesh: {len(m.cells)} cells, {len(m.faces)} faces, {len(m.points)} nodes "
         f"({time.time() - t0:.1f}s)")
     return fv, m, dyn, c, f, n, dict(dim_multiplier=mult, dt=dyn.delta_t, tau=tau)
+
+
+def _moments(env):
+    """Cell rho and u of the current pdf (the library's kernel on the tuned
+    handle, the host for the any-lattice handle)."""
+    if not getattr(env, "_lattice", False):
+        return env.cell_fields()
+    pdf = env.cells.pdf.astype(np.float64)
+    rho = pdf.sum(1)
+    return rho, (pdf @ np.asarray(env.cells.dynamics.KSI, np.float64)) / rho[:, None]
 
 
 def cpu_baseline(m, dyn, c, f, n, budget_s=12.0, threads=None):
@@ -105,13 +123,14 @@ def cpu_baseline(m, dyn, c, f, n, budget_s=12.0, threads=None):
                face_cells=f.stencil_cells_index, face_dists=f.stencil_dists,
                face_nodes=f.nodes_index, face_n=f.n, face_L=np.asarray(f.L).reshape(-1),
                node_cells=n.cells_index, node_dists=n.cell_dists)
-    st = oracle.initial_state(env, n.type, n.__dict__["_host_rho"], n.__dict__["_host_vel"], "f32")
+    Q = int(dyn.NUM_QUIVERS)
+    st = oracle.initial_state(env, n.type, n.__dict__["_host_rho"], n.__dict__["_host_vel"], "f32", Q)
     t0 = time.perf_counter()
-    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f32")
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f32", quivers=Q)
     one = time.perf_counter() - t0
     steps = max(1, min(200, int(budget_s / max(one, 1e-6))))
     t0 = time.perf_counter()
-    oracle.run(env, st, dyn.tau, dyn.delta_t, steps, "f32")
+    oracle.run(env, st, dyn.tau, dyn.delta_t, steps, "f32", quivers=Q)
     el = time.perf_counter() - t0
     C = c.face_indices.shape[0]
     return dict(value=round(C * steps / el / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
@@ -236,6 +255,8 @@ def main():
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--lattice", default="d2q9", choices=["d2q9", "d2q13"],
+                    help="d2q13: the any-lattice kernels (no batched or steady-state parts)")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--reorder", type=int, default=None)
@@ -256,6 +277,8 @@ def main():
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.lattice != "d2q9":  # the batched and steady-state parts run on D2Q9 handles
+        args.steady_cases = args.batch_cases = 0
     steady_cases = None
     if args.steady_cases > 0 and os.path.exists(args.weights):
         # meshed in a fork pool before anything touches the GPU
@@ -282,7 +305,8 @@ def main():
             t = _lib.copy_probe(mib << 20, 50)
             copy[f"GBps_{mib}MiB"] = round(2 * (mib << 20) / (t / 1e3) / 1e9, 1)
         log(f"copy probe by size: {copy}")
-    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache)
+    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache,
+                                           lattice=args.lattice)
     C = len(m.cells)
     env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block,
                          reorder=args.reorder)
@@ -294,16 +318,23 @@ def main():
     env.step(args.warmup)
     env.synchronize()
     ms = env.time_steps(args.steps)
-    r = env.step_residual(1)
-    if not np.isfinite(r):
-        raise SystemExit("non-finite residual: the run diverged")
+    if getattr(env, "_lattice", False):  # the any-lattice handle has no residual kernel
+        env.step(1)
+        r = None
+        if not np.isfinite(env.cells.pdf).all():
+            raise SystemExit("non-finite pdf: the run diverged")
+    else:
+        r = env.step_residual(1)
+        if not np.isfinite(r):
+            raise SystemExit("non-finite residual: the run diverged")
     mlups = C * args.steps / (ms / 1e3) / 1e6
     # two-step tiles unless the handle runs one step per launch (variant bit 8)
     pairs = not (int(info["variant"]) & 8)
-    kernel = "k_step2" if pairs else "k_step"
+    lattice_handle = getattr(env, "_lattice", False)
+    kernel = "k_lat_step" if lattice_handle else ("k_step2" if pairs else "k_step")
     updates_per_launch = 2 if pairs else 1
     per_launch_s = ms / 1e3 / args.steps * updates_per_launch
-    bpl = BYTES_PER_LUP_F32 if args.precision == "fp32" else BYTES_PER_LUP_F64
+    bpl = bytes_per_lup(int(dyn.NUM_QUIVERS), args.precision)
     achieved = bpl * C * updates_per_launch / per_launch_s / 1e9
     workload = f"cylinder_channel_{C // 1000}k" if C < 999_500 else f"cylinder_channel_{round(C / 1e6)}M"
     tr = load_traffic("cylinder_channel_1M") if abs(C - 1_000_000) < 50_000 else None
@@ -317,12 +348,12 @@ def main():
     if args.precision == "fp32":
         # the FP32 error of this run: the same steps with the same kernels in FP64
         total = args.warmup + args.steps + 1
-        r32, u32 = env.cell_fields()
+        r32, u32 = _moments(env)
         e64 = fv.Environment(c, f, n, precision="fp64", variant=args.variant, block=args.block,
                              reorder=args.reorder)
         e64.init()
         e64.step(total)
-        r64, u64 = e64.cell_fields()
+        r64, u64 = _moments(e64)
         e64.close()
         du = np.linalg.norm(u32.astype(np.float64) - u64) / np.linalg.norm(u64)
         dr = np.linalg.norm(r32.astype(np.float64) - r64) / np.linalg.norm(r64)
@@ -349,7 +380,10 @@ def main():
         "config": {"workload": workload, "cells": C, "faces": len(m.faces), "nodes": len(m.points),
                    "scheme": "upwind", "tau": case["tau"], "dt": case["dt"],
                    "dim_multiplier": case["dim_multiplier"], "seed": args.seed,
-                   "kernel": ("k_step2 (two fused cell steps per launch)" if pairs else "k_step (one step per launch)"),
+                   "lattice": args.lattice,
+                   "kernel": ("k_lat_step (any-lattice kernels, one step per launch)" if lattice_handle
+                              else "k_step2 (two fused cell steps per launch)" if pairs
+                              else "k_step (one step per launch)"),
                    "bytes_per_lup": bpl,
                    "residual_after": r},
         "roofline": roof,
