@@ -106,21 +106,47 @@ __device__ __forceinline__ void macro(const T (&f)[QN], T &rho, T &ux, T &uy) {
 
 // (W rho) * ((((1 + cu/C^2) + cu^2/(2C^4)) - uu/(2C^2)) [D2Q13: + cu^3/(2C^6)
 // - 3 cu uu/(2C^4)]), cu = ksi.u and uu = u.u as 0-initialised sums,
-// cu^3 = cu*(cu*cu) (src/dynamics.py:70-74, 101-102)
+// cu^3 = cu*(cu*cu) (src/dynamics.py:70-74, 101-102).
+// Opposite velocities give exactly negated cu (round-to-nearest is
+// sign-symmetric), hence negated cu/C^2, cu^3/(2C^6) and 3 cu uu/(2C^4) and
+// the same cu^2/(2C^4): each quotient is formed once per pair of opposite
+// directions and each bracket is still summed in the reference's order for
+// every q. A literal per-q evaluation can differ only in the sign of a zero
+// cu, which leaves every bracket unchanged. q = 0 is evaluated literally.
 template <typename T, int QN>
 __device__ __forceinline__ void equilibrium(const Consts<T, QN> &k, T rho, T ux, T uy,
                                             T (&out)[QN]) {
+  constexpr int kPos[6] = {1, 2, 5, 6, 9, 10};   // and their opposites:
+  constexpr int kNeg[6] = {3, 4, 7, 8, 11, 12};  // ksi_neg = -ksi_pos
+  constexpr int kPairs = QN == 13 ? 6 : 4;
   T uu = T(0);
   uu += ux * ux;
   uu += uy * uy;
-#pragma unroll
-  for (int q = 0; q < QN; ++q) {
+  const T D = uu / k.C2x2;
+  {
     T cu = T(0);
-    cu += T(kx(q)) * ux;
-    cu += T(ky(q)) * uy;
-    T br = ((T(1) + cu / k.C2) + (cu * cu) / k.C4x2) - uu / k.C2x2;
+    cu += T(0) * ux;
+    cu += T(0) * uy;
+    T br = ((T(1) + cu / k.C2) + (cu * cu) / k.C4x2) - D;
     if constexpr (QN == 13) br = (br + (cu * (cu * cu)) / k.C6x2) - ((T(3) * cu) * uu) / k.C4x2;
-    out[q] = (k.W[q] * rho) * br;
+    out[0] = (k.W[0] * rho) * br;
+  }
+#pragma unroll
+  for (int i = 0; i < kPairs; ++i) {
+    const int qp = kPos[i], qn = kNeg[i];
+    T cu = T(0);
+    cu += T(kx(qp)) * ux;
+    cu += T(ky(qp)) * uy;
+    const T A = cu / k.C2, B = (cu * cu) / k.C4x2;
+    T bp = ((T(1) + A) + B) - D;
+    T bn = ((T(1) + (-A)) + B) - D;
+    if constexpr (QN == 13) {
+      const T E = (cu * (cu * cu)) / k.C6x2, G = ((T(3) * cu) * uu) / k.C4x2;
+      bp = (bp + E) - G;
+      bn = (bn + (-E)) - (-G);
+    }
+    out[qp] = (k.W[qp] * rho) * bp;
+    out[qn] = (k.W[qn] * rho) * bn;
   }
 }
 
@@ -136,6 +162,7 @@ struct Args {
   T *node_pdf, *node_rho, *node_vel;     // [Q][N], [N], [2][N]
   const int32_t *an_gid, *an_type, *an_cells;
   const T *an_w, *an_wsum, *an_rho, *an_ux, *an_uy;
+  const int32_t *block_tile;  // [grid] tile (block of cells) of each workgroup, -1 = none
 };
 
 template <typename T, int QN>
@@ -210,14 +237,19 @@ __global__ void k_lat_nodes(Args<T> a, Consts<T, QN> k) {
   }
 }
 
-// One cell per thread: the three faces in cells.face_indices order (ghost
+// One cell per thread, tiles of blockDim.x consecutive cells taken in the
+// XCD-aware order of block_tile (workgroups go round-robin to the 8 XCDs;
+// each XCD gets one contiguous range of tiles, so neighbour gathers mostly
+// hit its own L2). The three faces in cells.face_indices order (ghost
 // g = fbar + (fbar - f_self) * ratio from the two face-node pdfs), upwind
 // by the sign of ksi.n (SCHEME 0) or Lax-Wendroff with slot 0 upstream
 // (SCHEME 1), flux (pdf*phi)*(s*L) summed in face order, then
 // f + dt*(1/tau*(feq - f) - sum) into fdst.
 template <typename T, int QN, int SCHEME>
 __global__ void k_lat_step(Args<T> a, Consts<T, QN> k) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = a.block_tile[blockIdx.x];
+  if (tile < 0) return;
+  const int c = tile * blockDim.x + threadIdx.x;
   if (c >= a.C) return;
   const size_t Cp = a.Cp;
   T f[QN], acc[QN];
@@ -373,6 +405,8 @@ struct Handle {
   void *fc0 = nullptr, *fc1 = nullptr, *fn0 = nullptr, *fn1 = nullptr, *fd0 = nullptr,
        *fd1 = nullptr, *fnx = nullptr, *fny = nullptr, *fL = nullptr;
   void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
+  void *block_tile = nullptr;
+  int grid = 1;
   int cur = 0;
   bool has_prev = false, observed = false;
   int64_t steps = 0, device_bytes = 0;
@@ -441,6 +475,7 @@ static Args<T> make_args(const Handle &h, int src) {
   a.an_rho = static_cast<const T *>(h.an_rho);
   a.an_ux = static_cast<const T *>(h.an_ux);
   a.an_uy = static_cast<const T *>(h.an_uy);
+  a.block_tile = static_cast<const int32_t *>(h.block_tile);
   return a;
 }
 
@@ -452,7 +487,7 @@ static int launch_step(Handle &h, int src) {
   const Args<T> a = make_args<T>(h, src);
   const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
   if (h.nan > 0) hipLaunchKernelGGL((k_lat_nodes<T, QN>), grid1(h.nan), dim3(256), 0, h.stream, a, k);
-  const dim3 grid((unsigned)std::max(1, (h.C + h.block - 1) / h.block));
+  const dim3 grid((unsigned)h.grid);
   if (h.scheme == FVDBM_UPWIND)
     hipLaunchKernelGGL((k_lat_step<T, QN, 0>), grid, dim3(h.block), 0, h.stream, a, k);
   else
@@ -583,6 +618,24 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
   if (!rc) rc = h.upload(&h.fnx, L.fnx);
   if (!rc) rc = h.upload(&h.fny, L.fny);
   if (!rc) rc = h.upload(&h.fL, L.fL);
+  if (!rc) {
+    // workgroup -> tile: workgroup b runs on XCD b % 8 and takes the
+    // (b / 8)-th tile of that XCD's contiguous range, tiles with boundary
+    // nodes first
+    const int per = (L.ntiles + 7) / 8;
+    std::vector<int32_t> bt(8 * (size_t)std::max(per, 1), -1);
+    for (int x = 0; x < 8; ++x) {
+      const int lo = std::min(x * per, L.ntiles), hi = std::min((x + 1) * per, L.ntiles);
+      std::vector<int32_t> order;
+      for (int t = lo; t < hi; ++t)
+        if (L.tile_node_off[t + 1] > L.tile_node_off[t]) order.push_back(t);
+      for (int t = lo; t < hi; ++t)
+        if (L.tile_node_off[t + 1] == L.tile_node_off[t]) order.push_back(t);
+      for (size_t i = 0; i < order.size(); ++i) bt[i * 8 + x] = order[i];
+    }
+    h.grid = (int)bt.size();
+    rc = h.upload(&h.block_tile, bt);
+  }
   if (rc) return rc;
   // node rho/vel hold the boundary-condition values (reference node order)
   std::vector<T> rho(h.N), vel(2 * (size_t)h.N);
