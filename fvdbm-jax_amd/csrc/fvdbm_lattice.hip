@@ -1,9 +1,9 @@
 // FVDBM step for any supported lattice: D2Q13 (src/dynamics.py:77-102), and
 // D2Q9 through the same code as a cross-check of the tuned D2Q9 kernels in
-// fvdbm_step.hip. One step is two plain kernels: the active boundary nodes
-// (Nodes.calc_pdfs, src/containers.py:339-390) into the node pdf array, then
-// one thread per cell for the face fluxes and the BGK update
-// (src/containers.py:107-121, 191-287). Same reference operation order and
+// fvdbm_step.hip. One step is one kernel: one thread per cell for the face
+// fluxes and the BGK update (src/containers.py:107-121, 191-287), a tile
+// with ghost faces first evaluating the boundary nodes they read
+// (Nodes.calc_pdfs, src/containers.py:339-390) into LDS. Same reference operation order and
 // no contraction (-ffp-contract=off) as the tuned kernels, so D2Q9 results
 // equal theirs bit for bit and D2Q13 results equal the C oracle's.
 //
@@ -163,6 +163,10 @@ struct Args {
   const int32_t *an_gid, *an_type, *an_cells;
   const T *an_w, *an_wsum, *an_rho, *an_ux, *an_uy;
   const int32_t *block_tile;  // [grid] tile (block of cells) of each workgroup, -1 = none
+  // per-tile node tables: entries of tile t are [tile_node_off[t],
+  // tile_node_off[t+1]) of tile_nodes (>= 0 active node, < 0: -1 - node id
+  // with a stored pdf); bf_slot [2][nbf] = a ghost face's two entries
+  const int32_t *tile_node_off, *tile_nodes, *bf_slot;
 };
 
 template <typename T, int QN>
@@ -171,65 +175,98 @@ __device__ __forceinline__ void load_cell(const T *f, size_t Cp, int c, T (&out)
   for (int q = 0; q < QN; ++q) out[q] = f[(size_t)q * Cp + c];
 }
 
-// Active nodes from fsrc: type 1 (velocity) rho <- IDW(cell rho), type 2
-// (density) u <- IDW(cell u); pdf = calc_eq(rho, u) + IDW(f_c - feq_c).
-// Padding entries (weight 0) contribute +0, as x*0 does in the reference.
+// Active node `an` from fsrc: type 1 (velocity) rho <- IDW(cell rho), type
+// 2 (density) u <- IDW(cell u); pdf = calc_eq(rho, u) + IDW(f_c - feq_c)
+// (src/containers.py:339-390). Padding entries (weight 0) contribute +0,
+// as x*0 does in the reference. Sixteen lanes per node: lane j computes the
+// weighted contributions of the node's j-th listed cell (loads and
+// arithmetic in parallel), then every lane of the group adds them up in
+// list order through shuffles, so each sum is the reference's
+// 0-initialised ordered sum. Called by every lane of the wave in uniform
+// control flow (the shuffles); `active` false gives zeros.
+constexpr int kNodeLanes = 16;
+
 template <typename T, int QN>
-__global__ void k_lat_nodes(Args<T> a, Consts<T, QN> k) {
-  const int an = blockIdx.x * blockDim.x + threadIdx.x;
-  if (an >= a.nan) return;
-  const int type = a.an_type[an];
-  const int32_t *cells = a.an_cells + (size_t)an * a.K;
-  const T *w = a.an_w + (size_t)an * a.K;
-  const T wsum = a.an_wsum[an];
-  T rho_n = a.an_rho[an], ux_n = a.an_ux[an], uy_n = a.an_uy[an];
-  T n0 = T(0), n1 = T(0);
-  for (int j = 0; j < a.K; ++j) {
-    const int cj = cells[j];
-    const T wj = w[j];
-    T r = T(0), ux = T(0), uy = T(0);
-    if (!(cj < 0 && wj == T(0))) {
-      T fc[QN];
-      load_cell<T, QN>(a.fsrc, a.Cp, cj < 0 ? -2 - cj : cj, fc);
-      macro<T, QN>(fc, r, ux, uy);
+__device__ __forceinline__ void group_node(const Args<T> &a, const Consts<T, QN> &k, int an,
+                                           bool active, T (&out)[QN], T &rho_n, T &ux_n,
+                                           T &uy_n) {
+  const int lane = threadIdx.x & (kNodeLanes - 1);
+  const int base = (threadIdx.x & 63) & ~(kNodeLanes - 1);  // the group's first lane in the wave
+  const int type = active ? a.an_type[an] : 1;
+  const int32_t *cells = a.an_cells + (size_t)(active ? an : 0) * a.K;
+  const T *w = a.an_w + (size_t)(active ? an : 0) * a.K;
+  T n0 = T(0), n1 = T(0), acc[QN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) acc[q] = T(0);
+  for (int j0 = 0; j0 < a.K; j0 += kNodeLanes) {
+    const int j = j0 + lane;
+    T vr = T(0), vx = T(0), vy = T(0), vq[QN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) vq[q] = T(0);
+    if (active && j < a.K) {
+      const int cj = cells[j];
+      const T wj = w[j];
+      T r = T(0), ux = T(0), uy = T(0), fc[QN];
+      if (!(cj < 0 && wj == T(0))) {
+        load_cell<T, QN>(a.fsrc, a.Cp, cj < 0 ? -2 - cj : cj, fc);
+        macro<T, QN>(fc, r, ux, uy);
+      }
+      vr = r * wj;
+      vx = ux * wj;
+      vy = uy * wj;
+      if (cj < 0) {
+#pragma unroll
+        for (int q = 0; q < QN; ++q) vq[q] = T(0) * wj;
+      } else {
+        T eq[QN];
+        equilibrium<T, QN>(k, r, ux, uy, eq);
+#pragma unroll
+        for (int q = 0; q < QN; ++q) vq[q] = (fc[q] - eq[q]) * wj;
+      }
     }
-    if (type == 1) {
-      n0 += r * wj;
-    } else {
-      n0 += ux * wj;
-      n1 += uy * wj;
+    const int nj = a.K - j0 < kNodeLanes ? a.K - j0 : kNodeLanes;
+    for (int t = 0; t < nj; ++t) {
+      const int src = base + t;
+      const T sr = __shfl(vr, src), sx = __shfl(vx, src), sy = __shfl(vy, src);
+      if (type == 1) {
+        n0 += sr;
+      } else {
+        n0 += sx;
+        n1 += sy;
+      }
+#pragma unroll
+      for (int q = 0; q < QN; ++q) acc[q] += __shfl(vq[q], src);
     }
   }
+  const T wsum = active ? a.an_wsum[an] : T(1);
+  rho_n = active ? a.an_rho[an] : T(1);
+  ux_n = active ? a.an_ux[an] : T(0);
+  uy_n = active ? a.an_uy[an] : T(0);
   if (type == 1) {
     rho_n = n0 / wsum;
   } else {
     ux_n = n0 / wsum;
     uy_n = n1 / wsum;
   }
-  T acc[QN];
-#pragma unroll
-  for (int q = 0; q < QN; ++q) acc[q] = T(0);
-  for (int j = 0; j < a.K; ++j) {
-    const int cj = cells[j];
-    const T wj = w[j];
-    if (cj < 0) {
-#pragma unroll
-      for (int q = 0; q < QN; ++q) acc[q] += T(0) * wj;
-    } else {
-      T fc[QN], eq[QN], r, ux, uy;
-      load_cell<T, QN>(a.fsrc, a.Cp, cj, fc);
-      macro<T, QN>(fc, r, ux, uy);
-      equilibrium<T, QN>(k, r, ux, uy, eq);
-#pragma unroll
-      for (int q = 0; q < QN; ++q) acc[q] += (fc[q] - eq[q]) * wj;
-    }
-  }
   T eqn[QN];
   equilibrium<T, QN>(k, rho_n, ux_n, uy_n, eqn);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) out[q] = eqn[q] + acc[q] / wsum;
+}
+
+// Every active node from fsrc into the node arrays (the observables; the
+// step computes the values it needs per tile). 16 nodes per 256 threads.
+template <typename T, int QN>
+__global__ void __launch_bounds__(256) k_lat_nodes(Args<T> a, Consts<T, QN> k) {
+  const int an = blockIdx.x * (blockDim.x / kNodeLanes) + (int)(threadIdx.x / kNodeLanes);
+  const bool active = an < a.nan;  // uniform per group; no return before the shuffles
+  T out[QN], rho_n, ux_n, uy_n;
+  group_node<T, QN>(a, k, an, active, out, rho_n, ux_n, uy_n);
+  if (!active || (threadIdx.x & (kNodeLanes - 1)) != 0) return;
   const int g = a.an_gid[an];
 #pragma unroll
-  for (int q = 0; q < QN; ++q) a.node_pdf[(size_t)q * a.N + g] = eqn[q] + acc[q] / wsum;
-  if (type == 1) {
+  for (int q = 0; q < QN; ++q) a.node_pdf[(size_t)q * a.N + g] = out[q];
+  if (a.an_type[an] == 1) {
     a.node_rho[g] = rho_n;
   } else {
     a.node_vel[g] = ux_n;
@@ -240,17 +277,38 @@ __global__ void k_lat_nodes(Args<T> a, Consts<T, QN> k) {
 // One cell per thread, tiles of blockDim.x consecutive cells taken in the
 // XCD-aware order of block_tile (workgroups go round-robin to the 8 XCDs;
 // each XCD gets one contiguous range of tiles, so neighbour gathers mostly
-// hit its own L2). The three faces in cells.face_indices order (ghost
+// hit its own L2). A tile with ghost faces first evaluates the distinct
+// face nodes they read into LDS (active nodes by group_node, stored pdfs
+// of the others). The three faces in cells.face_indices order (ghost
 // g = fbar + (fbar - f_self) * ratio from the two face-node pdfs), upwind
 // by the sign of ksi.n (SCHEME 0) or Lax-Wendroff with slot 0 upstream
 // (SCHEME 1), flux (pdf*phi)*(s*L) summed in face order, then
 // f + dt*(1/tau*(feq - f) - sum) into fdst.
 template <typename T, int QN, int SCHEME>
-__global__ void k_lat_step(Args<T> a, Consts<T, QN> k) {
+__global__ void __launch_bounds__(1024) k_lat_step(Args<T> a, Consts<T, QN> k) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *tab = reinterpret_cast<T *>(smem_raw);  // [QN][ntn] node pdfs of the tile's ghost faces
   const int tile = a.block_tile[blockIdx.x];
-  if (tile < 0) return;
+  if (tile < 0) return;  // uniform per block
+  const int t0 = a.tile_node_off[tile], ntn = a.tile_node_off[tile + 1] - t0;
+  if (ntn > 0) {  // uniform per block: the node values this tile's ghost faces read
+    const int groups = blockDim.x / kNodeLanes, group = threadIdx.x / kNodeLanes;
+    const bool lead = (threadIdx.x & (kNodeLanes - 1)) == 0;
+    for (int i0 = 0; i0 < ntn; i0 += groups) {
+      const int i = i0 + group;
+      const int e = i < ntn ? a.tile_nodes[t0 + i] : -1;
+      T out[QN], r, ux, uy;
+      group_node<T, QN>(a, k, e, i < ntn && e >= 0, out, r, ux, uy);
+      if (lead && i < ntn) {
+#pragma unroll
+        for (int q = 0; q < QN; ++q)
+          tab[q * ntn + i] = e >= 0 ? out[q] : a.node_pdf[(size_t)q * a.N + (-1 - e)];
+      }
+    }
+    __syncthreads();
+  }
   const int c = tile * blockDim.x + threadIdx.x;
-  if (c >= a.C) return;
+  if (c >= a.C) return;  // after the only barrier
   const size_t Cp = a.Cp;
   T f[QN], acc[QN];
   load_cell<T, QN>(a.fsrc, Cp, c, f);
@@ -267,10 +325,10 @@ __global__ void k_lat_step(Args<T> a, Consts<T, QN> k) {
     } else {
       const int bf = code & kIndexMask;
       const T ratio = a.bf_ratio[bf];
-      const int g0 = a.bf_gnode[bf], g1 = a.bf_gnode[a.nbf + bf];
+      const int s0 = a.bf_slot[bf], s1 = a.bf_slot[a.nbf + bf];
 #pragma unroll
       for (int q = 0; q < QN; ++q) {
-        const T fbar = (a.node_pdf[(size_t)q * a.N + g0] + a.node_pdf[(size_t)q * a.N + g1]) / T(2);
+        const T fbar = (tab[q * ntn + s0] + tab[q * ntn + s1]) / T(2);
         o[q] = fbar + (fbar - f[q]) * ratio;
       }
     }
@@ -405,8 +463,8 @@ struct Handle {
   void *fc0 = nullptr, *fc1 = nullptr, *fn0 = nullptr, *fn1 = nullptr, *fd0 = nullptr,
        *fd1 = nullptr, *fnx = nullptr, *fny = nullptr, *fL = nullptr;
   void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
-  void *block_tile = nullptr;
-  int grid = 1;
+  void *block_tile = nullptr, *tile_node_off = nullptr, *tile_nodes = nullptr, *bf_slot = nullptr;
+  int grid = 1, max_tile_nodes = 0;
   int cur = 0;
   bool has_prev = false, observed = false;
   int64_t steps = 0, device_bytes = 0;
@@ -476,6 +534,9 @@ static Args<T> make_args(const Handle &h, int src) {
   a.an_ux = static_cast<const T *>(h.an_ux);
   a.an_uy = static_cast<const T *>(h.an_uy);
   a.block_tile = static_cast<const int32_t *>(h.block_tile);
+  a.tile_node_off = static_cast<const int32_t *>(h.tile_node_off);
+  a.tile_nodes = static_cast<const int32_t *>(h.tile_nodes);
+  a.bf_slot = static_cast<const int32_t *>(h.bf_slot);
   return a;
 }
 
@@ -486,12 +547,12 @@ template <typename T, int QN>
 static int launch_step(Handle &h, int src) {
   const Args<T> a = make_args<T>(h, src);
   const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
-  if (h.nan > 0) hipLaunchKernelGGL((k_lat_nodes<T, QN>), grid1(h.nan), dim3(256), 0, h.stream, a, k);
   const dim3 grid((unsigned)h.grid);
+  const size_t lds = (size_t)QN * h.max_tile_nodes * sizeof(T);
   if (h.scheme == FVDBM_UPWIND)
-    hipLaunchKernelGGL((k_lat_step<T, QN, 0>), grid, dim3(h.block), 0, h.stream, a, k);
+    hipLaunchKernelGGL((k_lat_step<T, QN, 0>), grid, dim3(h.block), lds, h.stream, a, k);
   else
-    hipLaunchKernelGGL((k_lat_step<T, QN, 1>), grid, dim3(h.block), 0, h.stream, a, k);
+    hipLaunchKernelGGL((k_lat_step<T, QN, 1>), grid, dim3(h.block), lds, h.stream, a, k);
   LHIP_TRY(hipGetLastError());
   return FVDBM_OK;
 }
@@ -618,6 +679,12 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
   if (!rc) rc = h.upload(&h.fnx, L.fnx);
   if (!rc) rc = h.upload(&h.fny, L.fny);
   if (!rc) rc = h.upload(&h.fL, L.fL);
+  if (!rc) rc = h.upload(&h.tile_node_off, L.tile_node_off);
+  if (!rc) rc = h.upload(&h.tile_nodes, L.tile_nodes);
+  if (!rc) rc = h.upload(&h.bf_slot, L.bf_slot);
+  h.max_tile_nodes = L.max_tile_nodes;
+  if ((size_t)QN * h.max_tile_nodes * sizeof(T) > 65536)
+    return fail(FVDBM_ERR_INVALID, "a tile's boundary-node table exceeds the LDS of a block");
   if (!rc) {
     // workgroup -> tile: workgroup b runs on XCD b % 8 and takes the
     // (b / 8)-th tile of that XCD's contiguous range, tiles with boundary
@@ -663,6 +730,9 @@ static int observe(Handle &h) {
   if (rc) return rc;
   const Args<T> a = make_args<T>(h, h.cur ^ 1);
   const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+  if (h.nan > 0)  // node values at the start of the last step; 16 nodes per block
+    hipLaunchKernelGGL((k_lat_nodes<T, QN>), dim3((unsigned)((h.nan + 15) / 16)), dim3(256), 0,
+                       h.stream, a, k);
   hipLaunchKernelGGL((k_lat_moments<T, QN>), grid1(h.C), dim3(256), 0, h.stream, a, k,
                      static_cast<T *>(h.obs_rho), static_cast<T *>(h.obs_vel),
                      static_cast<T *>(h.obs_feq));
@@ -745,12 +815,15 @@ static int get_typed(Handle &h, int field, void *dst) {
       return FVDBM_OK;
     }
     case FVDBM_NODE_PDF:
-      return nodes(h.node_pdf, QN);
     case FVDBM_NODE_RHO:
+    case FVDBM_NODE_VEL: {
+      int rc = observe<T, QN>(h);
+      if (rc) return rc;
+      if (field == FVDBM_NODE_PDF) return nodes(h.node_pdf, QN);
+      if (field == FVDBM_NODE_VEL) return nodes(h.node_vel, 2);
       LHIP_TRY(hipMemcpy(out, h.node_rho, (size_t)h.N * es, hipMemcpyDeviceToHost));
       return FVDBM_OK;
-    case FVDBM_NODE_VEL:
-      return nodes(h.node_vel, 2);
+    }
     default:
       return fail(FVDBM_ERR_INVALID, "unknown field");
   }
