@@ -331,7 +331,7 @@ def main():
     # two-step tiles unless the handle runs one step per launch (variant bit 8)
     pairs = not (int(info["variant"]) & 8)
     lattice_handle = getattr(env, "_lattice", False)
-    kernel = "k_lat_step" if lattice_handle else ("k_step2" if pairs else "k_step")
+    kernel = ("k_lat_step2" if pairs else "k_lat_step") if lattice_handle else ("k_step2" if pairs else "k_step")
     updates_per_launch = 2 if pairs else 1
     per_launch_s = ms / 1e3 / args.steps * updates_per_launch
     bpl = bytes_per_lup(int(dyn.NUM_QUIVERS), args.precision)
@@ -381,9 +381,10 @@ def main():
                    "scheme": "upwind", "tau": case["tau"], "dt": case["dt"],
                    "dim_multiplier": case["dim_multiplier"], "seed": args.seed,
                    "lattice": args.lattice,
-                   "kernel": ("k_lat_step (any-lattice kernels, one step per launch)" if lattice_handle
-                              else "k_step2 (two fused cell steps per launch)" if pairs
-                              else "k_step (one step per launch)"),
+                   "kernel": {"k_lat_step2": "k_lat_step2 (any-lattice kernels, two cell steps per launch)",
+                              "k_lat_step": "k_lat_step (any-lattice kernels, one step per launch)",
+                              "k_step2": "k_step2 (two fused cell steps per launch)",
+                              "k_step": "k_step (one step per launch)"}[kernel],
                    "bytes_per_lup": bpl,
                    "residual_after": r},
         "roofline": roof,

@@ -363,6 +363,269 @@ __global__ void __launch_bounds__(1024) k_lat_step(Args<T> a, Consts<T, QN> k) {
     a.fdst[(size_t)q * Cp + c] = f[q] + k.dt * (k.invtau * (eq[q] - f[q]) - acc[q]);
 }
 
+// ----------------------------------------------------------------------------
+// two-step tiles (upwind): one launch advances a tile by two steps
+// ----------------------------------------------------------------------------
+// The ring lists, ring face records and extended node tables come from the
+// host layout builder (the tuned D2Q9 kernel uses the same ones): with O a
+// tile's cells, S1 = O, its face neighbours and every cell a node of an O
+// ghost face reads, S2 = the face neighbours of S1 outside it. Step n runs
+// for S1 from the inputs of S1 and S2 staged in LDS, the results stay in
+// LDS, then step n + 1 runs for O. Every value equals what two k_lat_step
+// launches compute.
+template <typename T>
+struct PArgs {
+  const int32_t *code2;                    // [3][Cp] own face codes with stage slots
+  const int32_t *ring_off, *ring_cells, *ring_n1, *rg_off;
+  const int32_t *rg_code;                  // [3][nrgc] face records of the S1 \ O cells
+  const T *rg_nx, *rg_ny, *rg_sL;
+  const int32_t *rgb_slot;                 // [2][nrgb] extended-table slots of ring ghosts
+  const T *rgb_ratio;
+  const int32_t *xnode_off, *xnodes, *xentry_pair_off, *xpair_off, *xpair_cell;
+  const T *xpair_w;
+  const int32_t *entry_pair_off, *tile_pair_off, *pair_cell, *pair_slot;
+  const T *pair_w;
+  int nrgc, nrgb, max_xnodes, max_xpairs;
+};
+
+// Node table entries [0, nent) into tab (rows of length nrow) from their
+// (node, cell) pairs [pb, pb + np) (offsets epo absolute): one thread per
+// pair puts the pair's weighted contributions [r w, ux w, uy w, (f - feq) w]
+// (its cell read from fsrc, or from the stage slot pslot when FROM_LDS)
+// into an LDS record of PairRec<QN>::W words, then one thread per entry
+// adds its pairs' records in pair order: group_node's arithmetic (pairs of
+// weight-0 padding are left out: they add +0). Ends with a barrier.
+template <int QN>
+struct PairRec {
+  static constexpr int W = 3 + QN + ((3 + QN) % 2 == 0 ? 1 : 0);  // odd stride: spread LDS banks
+};
+
+template <typename T, int QN, bool FROM_LDS>
+__device__ __forceinline__ void lat_table_rec(const Args<T> &a, const Consts<T, QN> &k, T *tab,
+                                              int nrow, T *con,
+                                              const int32_t *__restrict__ entries, int nent,
+                                              const int32_t *__restrict__ epo, int pb, int np,
+                                              const int32_t *__restrict__ pcell,
+                                              const int32_t *__restrict__ pslot,
+                                              const T *__restrict__ pw, const T *stage) {
+  constexpr int RW = PairRec<QN>::W;
+  const int bs = blockDim.x;
+  for (int pi = threadIdx.x; pi < np; pi += bs) {
+    const int cj = pcell[pb + pi];
+    const T wj = pw[pb + pi];
+    T fc[QN], r, ux, uy;
+    if constexpr (FROM_LDS) {
+      const int sl = pslot[pb + pi];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) fc[q] = stage[sl * QN + q];
+    } else {
+      load_cell<T, QN>(a.fsrc, a.Cp, cj < 0 ? -2 - cj : cj, fc);
+    }
+    macro<T, QN>(fc, r, ux, uy);
+    T *o = con + RW * pi;
+    o[0] = r * wj;
+    o[1] = ux * wj;
+    o[2] = uy * wj;
+    if (cj >= 0) {
+      T eq[QN];
+      equilibrium<T, QN>(k, r, ux, uy, eq);
+#pragma unroll
+      for (int q = 0; q < QN; ++q) o[3 + q] = (fc[q] - eq[q]) * wj;
+    } else {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) o[3 + q] = T(0) * wj;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nent; i += bs) {
+    const int e = entries[i];
+    T out[QN];
+    if (e >= 0) {
+      const int type = a.an_type[e];
+      const int p0 = epo[i] - pb, p1 = epo[i + 1] - pb;
+      T n0 = T(0), n1 = T(0), acc[QN];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) acc[q] = T(0);
+      for (int pi = p0; pi < p1; ++pi) {
+        const T *o = con + RW * pi;
+        if (type == 1) {
+          n0 += o[0];
+        } else {
+          n0 += o[1];
+          n1 += o[2];
+        }
+#pragma unroll
+        for (int q = 0; q < QN; ++q) acc[q] += o[3 + q];
+      }
+      const T wsum = a.an_wsum[e];
+      T rho_n = a.an_rho[e], ux_n = a.an_ux[e], uy_n = a.an_uy[e];
+      if (type == 1) {
+        rho_n = n0 / wsum;
+      } else {
+        ux_n = n0 / wsum;
+        uy_n = n1 / wsum;
+      }
+      T eqn[QN];
+      equilibrium<T, QN>(k, rho_n, ux_n, uy_n, eqn);
+#pragma unroll
+      for (int q = 0; q < QN; ++q) out[q] = eqn[q] + acc[q] / wsum;
+    } else {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) out[q] = a.node_pdf[(size_t)q * a.N + (-1 - e)];
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) tab[q * nrow + i] = out[q];
+  }
+  __syncthreads();
+}
+
+// One upwind update of a cell whose neighbours are in the stage ([slot][QN])
+// and whose ghost-face nodes are in tab: k_lat_step's arithmetic.
+template <typename T, int QN>
+__device__ __forceinline__ void lat_update_staged(const Consts<T, QN> &k, const T (&f)[QN],
+                                                  const int32_t (&code)[3], const T (&nx)[3],
+                                                  const T (&ny)[3], const T (&sL)[3],
+                                                  const T *stage, const T *tab, int nrow,
+                                                  const int32_t *__restrict__ gslot, int gstride,
+                                                  const T *__restrict__ gratio, T (&fn)[QN]) {
+  T acc[QN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) acc[q] = T(0);
+#pragma unroll
+  for (int kf = 0; kf < 3; ++kf) {
+    const bool slot1 = (code[kf] & kSlot1Bit) != 0;
+    T o[QN];
+    if (code[kf] >= 0) {
+      const int sl = code[kf] & kIndexMask;
+#pragma unroll
+      for (int q = 0; q < QN; ++q) o[q] = stage[sl * QN + q];
+    } else {
+      const int bf = code[kf] & kIndexMask;
+      const T ratio = gratio[bf];
+      const int s0 = gslot[bf], s1 = gslot[gstride + bf];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
+        o[q] = fbar + (fbar - f[q]) * ratio;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      T phi = T(0);
+      phi += T(kx(q)) * nx[kf];
+      phi += T(ky(q)) * ny[kf];
+      const bool take_slot0 = phi >= T(0);
+      const T up = (take_slot0 != slot1) ? f[q] : o[q];
+      acc[q] += (up * phi) * sL[kf];
+    }
+  }
+  T rho, ux, uy, eq[QN];
+  macro<T, QN>(f, rho, ux, uy);
+  equilibrium<T, QN>(k, rho, ux, uy, eq);
+#pragma unroll
+  for (int q = 0; q < QN; ++q) fn[q] = f[q] + k.dt * (k.invtau * (eq[q] - f[q]) - acc[q]);
+}
+
+template <typename T, int QN>
+__global__ void __launch_bounds__(1024) k_lat_step2(Args<T> a, PArgs<T> p, Consts<T, QN> k) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *tab = reinterpret_cast<T *>(smem_raw);              // [QN][nx] extended node table
+  T *con = tab + QN * p.max_xnodes;                      // pair records of a node table
+  T *stage = con + PairRec<QN>::W * p.max_xpairs;        // [slot][QN]: O, then S1 \ O and S2
+  const int tile = a.block_tile[blockIdx.x];
+  if (tile < 0) return;  // uniform per block
+  const int bs = blockDim.x;
+  const size_t Cp = a.Cp;
+  const int c = tile * bs + threadIdx.x;
+  const bool live = c < a.C;
+  const int x0 = p.xnode_off[tile], nx = p.xnode_off[tile + 1] - x0;
+  const int t0 = a.tile_node_off[tile], ntn = a.tile_node_off[tile + 1] - t0;
+  // step-n node values of every ghost face of S1, from global memory
+  if (nx > 0) {
+    const int pb = p.xpair_off[tile];
+    lat_table_rec<T, QN, false>(a, k, tab, nx, con, p.xnodes + x0, nx, p.xentry_pair_off + x0, pb,
+                                p.xpair_off[tile + 1] - pb, p.xpair_cell, nullptr, p.xpair_w, stage);
+  }
+  // stage: own cells, then S1 \ O and S2; face records in registers
+  T f[QN];
+  int32_t cd[3] = {0, 0, 0};
+  T fnx[3] = {}, fny[3] = {}, fsl[3] = {};
+  if (live) {
+    load_cell<T, QN>(a.fsrc, Cp, c, f);
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) {
+      const size_t at = (size_t)kf * Cp + c;
+      cd[kf] = p.code2[at];
+      fnx[kf] = a.nx[at];
+      fny[kf] = a.ny[at];
+      fsl[kf] = a.sL[at];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < QN; ++q) f[q] = T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < QN; ++q) stage[threadIdx.x * QN + q] = f[q];
+  const int r0 = p.ring_off[tile], nr = p.ring_off[tile + 1] - r0, n1 = p.ring_n1[tile];
+  for (int i = threadIdx.x; i < nr; i += bs) {
+    T g[QN];
+    load_cell<T, QN>(a.fsrc, Cp, p.ring_cells[r0 + i], g);
+#pragma unroll
+    for (int q = 0; q < QN; ++q) stage[(bs + i) * QN + q] = g[q];
+  }
+  const bool ring_here = (int)threadIdx.x < n1;
+  int32_t rc[3] = {0, 0, 0};
+  T rnx[3] = {}, rny[3] = {}, rsl[3] = {};
+  if (ring_here) {
+    const int rr = p.rg_off[tile] + threadIdx.x;
+#pragma unroll
+    for (int kf = 0; kf < 3; ++kf) {
+      const size_t at = (size_t)kf * p.nrgc + rr;
+      rc[kf] = p.rg_code[at];
+      rnx[kf] = p.rg_nx[at];
+      rny[kf] = p.rg_ny[at];
+      rsl[kf] = p.rg_sL[at];
+    }
+  }
+  __syncthreads();
+  // step n for the own cell and the ring cell of this thread
+  T fo[QN], fq[QN];
+  if (live)
+    lat_update_staged<T, QN>(k, f, cd, fnx, fny, fsl, stage, tab, nx, a.bf_slot, a.nbf, a.bf_ratio,
+                             fo);
+  if (ring_here) {
+    T g[QN];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) g[q] = stage[(bs + threadIdx.x) * QN + q];
+    lat_update_staged<T, QN>(k, g, rc, rnx, rny, rsl, stage, tab, nx, p.rgb_slot, p.nrgb,
+                             p.rgb_ratio, fq);
+  }
+  __syncthreads();  // every step-n input has been read
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < QN; ++q) stage[threadIdx.x * QN + q] = fo[q];
+  }
+  if (ring_here) {
+#pragma unroll
+    for (int q = 0; q < QN; ++q) stage[(bs + threadIdx.x) * QN + q] = fq[q];
+  }
+  __syncthreads();
+  // step-(n+1) node values of the own ghost faces, from the stage
+  if (ntn > 0) {
+    const int pb = p.tile_pair_off[tile];
+    lat_table_rec<T, QN, true>(a, k, tab, nx, con, a.tile_nodes + t0, ntn, p.entry_pair_off + t0,
+                               pb, p.tile_pair_off[tile + 1] - pb, p.pair_cell, p.pair_slot,
+                               p.pair_w, stage);
+  }
+  if (live) {
+    T fn[QN];
+    lat_update_staged<T, QN>(k, fo, cd, fnx, fny, fsl, stage, tab, nx, a.bf_slot, a.nbf, a.bf_ratio,
+                             fn);
+#pragma unroll
+    for (int q = 0; q < QN; ++q) a.fdst[(size_t)q * Cp + c] = fn[q];
+  }
+}
+
 // cells.rho / vel / pdf_eq of fsrc (device order)
 template <typename T, int QN>
 __global__ void k_lat_moments(Args<T> a, Consts<T, QN> k, T *rho, T *vel, T *feq) {
@@ -465,6 +728,18 @@ struct Handle {
   void *obs_rho = nullptr, *obs_vel = nullptr, *obs_feq = nullptr, *obs_flux = nullptr;
   void *block_tile = nullptr, *tile_node_off = nullptr, *tile_nodes = nullptr, *bf_slot = nullptr;
   int grid = 1, max_tile_nodes = 0;
+  // two-step tiles (upwind, when the layout's rings fit and the LDS does)
+  bool pairs = false, two_step = true;  // two_step: wanted (params.variant bit 8 clear)
+  size_t lds2 = 0;
+  int nrgc = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0;
+  void *code2 = nullptr, *ring_off = nullptr, *ring_cells = nullptr, *ring_n1 = nullptr,
+       *rg_off = nullptr, *rg_code = nullptr, *rg_nx = nullptr, *rg_ny = nullptr, *rg_sL = nullptr,
+       *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr, *xnodes = nullptr,
+       *xentry_pair_off = nullptr, *xpair_off = nullptr, *xpair_cell = nullptr, *xpair_w = nullptr,
+       *entry_pair_off = nullptr, *tile_pair_off = nullptr, *pair_cell = nullptr,
+       *pair_slot = nullptr, *pair_w = nullptr;
+  hipGraph_t graph2[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec2[2] = {nullptr, nullptr};
   int cur = 0;
   bool has_prev = false, observed = false;
   int64_t steps = 0, device_bytes = 0;
@@ -478,6 +753,8 @@ struct Handle {
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
+      if (gexec2[i]) (void)hipGraphExecDestroy(gexec2[i]);
+      if (graph2[i]) (void)hipGraphDestroy(graph2[i]);
     }
     for (void *b : bufs) (void)hipFree(b);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -572,6 +849,72 @@ static int step_any(Handle &h, int src) {
   });
 }
 
+// two steps reading f[src] and writing f[src ^ 1]
+template <typename T, int QN>
+static int launch_pair(Handle &h, int src) {
+  const Args<T> a = make_args<T>(h, src);
+  const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
+  PArgs<T> p;
+  p.code2 = static_cast<const int32_t *>(h.code2);
+  p.ring_off = static_cast<const int32_t *>(h.ring_off);
+  p.ring_cells = static_cast<const int32_t *>(h.ring_cells);
+  p.ring_n1 = static_cast<const int32_t *>(h.ring_n1);
+  p.rg_off = static_cast<const int32_t *>(h.rg_off);
+  p.rg_code = static_cast<const int32_t *>(h.rg_code);
+  p.rg_nx = static_cast<const T *>(h.rg_nx);
+  p.rg_ny = static_cast<const T *>(h.rg_ny);
+  p.rg_sL = static_cast<const T *>(h.rg_sL);
+  p.rgb_slot = static_cast<const int32_t *>(h.rgb_slot);
+  p.rgb_ratio = static_cast<const T *>(h.rgb_ratio);
+  p.xnode_off = static_cast<const int32_t *>(h.xnode_off);
+  p.xnodes = static_cast<const int32_t *>(h.xnodes);
+  p.xentry_pair_off = static_cast<const int32_t *>(h.xentry_pair_off);
+  p.xpair_off = static_cast<const int32_t *>(h.xpair_off);
+  p.xpair_cell = static_cast<const int32_t *>(h.xpair_cell);
+  p.xpair_w = static_cast<const T *>(h.xpair_w);
+  p.entry_pair_off = static_cast<const int32_t *>(h.entry_pair_off);
+  p.tile_pair_off = static_cast<const int32_t *>(h.tile_pair_off);
+  p.pair_cell = static_cast<const int32_t *>(h.pair_cell);
+  p.pair_slot = static_cast<const int32_t *>(h.pair_slot);
+  p.pair_w = static_cast<const T *>(h.pair_w);
+  p.nrgc = h.nrgc;
+  p.nrgb = h.nrgb;
+  p.max_xnodes = h.max_xnodes;
+  p.max_xpairs = h.max_xpairs;
+  hipLaunchKernelGGL((k_lat_step2<T, QN>), dim3((unsigned)h.grid), dim3(h.block), h.lds2, h.stream,
+                     a, p, k);
+  LHIP_TRY(hipGetLastError());
+  return FVDBM_OK;
+}
+
+static int pair_any(Handle &h, int src) {
+  return dispatch(h, [&](auto tp, auto qn) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_pair<T, decltype(qn)::value>(h, src);
+  });
+}
+
+// graph_steps / 2 two-step launches starting from `parity`
+static int ensure_graph2(Handle &h, int parity) {
+  if (h.gexec2[parity]) return FVDBM_OK;
+  LHIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  int src = parity, rc = FVDBM_OK;
+  for (int s = 0; s < h.graph_steps / 2 && rc == FVDBM_OK; ++s) {
+    rc = pair_any(h, src);
+    src ^= 1;
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h.graph2[parity] = g;
+  LHIP_TRY(hipGraphInstantiate(&h.gexec2[parity], g, nullptr, nullptr, 0));
+  return FVDBM_OK;
+}
+
 static int ensure_graph(Handle &h, int parity) {
   if (h.gexec[parity]) return FVDBM_OK;
   LHIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
@@ -594,6 +937,28 @@ static int ensure_graph(Handle &h, int parity) {
 
 static int run_steps(Handle &h, int64_t n) {
   if (n <= 0) return FVDBM_OK;
+  if (h.pairs && n >= 3) {
+    // pairs of steps for all but the last one or two, so the last step is
+    // a single launch and f[cur ^ 1] keeps the pdf at its start
+    int64_t rest = n - 1;
+    if (h.graph_steps >= 2) {
+      while (rest >= h.graph_steps) {
+        int rc = ensure_graph2(h, h.cur);
+        if (rc) return rc;
+        LHIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
+        h.cur ^= (h.graph_steps / 2) & 1;
+        rest -= h.graph_steps;
+        h.steps += h.graph_steps;
+      }
+    }
+    for (; rest >= 2; rest -= 2) {
+      int rc = pair_any(h, h.cur);
+      if (rc) return rc;
+      h.cur ^= 1;
+      h.steps += 2;
+    }
+    n = rest + 1;  // the remaining single steps, the last included
+  }
   if (h.graph_steps > 0) {
     while (n >= h.graph_steps) {
       int rc = ensure_graph(h, h.cur);
@@ -704,6 +1069,53 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
     rc = h.upload(&h.block_tile, bt);
   }
   if (rc) return rc;
+  // two-step tiles: upwind, rings within a tile's threads, LDS of a block
+  const size_t lds2 = ((size_t)QN * L.max_xnodes + (size_t)PairRec<QN>::W * L.max_xpairs +
+                       (size_t)QN * (h.block + L.max_ring)) * es;
+  if (h.two_step && h.scheme == FVDBM_UPWIND && L.pair_ok && lds2 <= 65536) {
+    const int nrgc = (int)(L.rg_code.size() / 3);
+    std::vector<int32_t> rcode(3 * (size_t)std::max(nrgc, 1), 0);
+    std::vector<T> rnx(rcode.size(), T(0)), rny(rcode.size(), T(0)), rsl(rcode.size(), T(0));
+    for (int r = 0; r < nrgc; ++r)
+      for (int kf = 0; kf < 3; ++kf) {
+        const size_t d = (size_t)kf * nrgc + r, src = 3 * (size_t)r + kf;
+        rcode[d] = L.rg_code[src];
+        rnx[d] = L.rg_nx[src];
+        rny[d] = L.rg_ny[src];
+        rsl[d] = L.rg_sL[src];
+      }
+    std::vector<int32_t> rg_off(L.ntiles, 0);
+    for (int t = 1; t < L.ntiles; ++t) rg_off[t] = rg_off[t - 1] + L.ring_n1[t - 1];
+    rc = h.upload(&h.code2, L.nb_slot);
+    if (!rc) rc = h.upload(&h.ring_off, L.ring_off);
+    if (!rc) rc = h.upload(&h.ring_cells, L.ring_cells);
+    if (!rc) rc = h.upload(&h.ring_n1, L.ring_n1);
+    if (!rc) rc = h.upload(&h.rg_off, rg_off);
+    if (!rc) rc = h.upload(&h.rg_code, rcode);
+    if (!rc) rc = h.upload(&h.rg_nx, rnx);
+    if (!rc) rc = h.upload(&h.rg_ny, rny);
+    if (!rc) rc = h.upload(&h.rg_sL, rsl);
+    if (!rc) rc = h.upload(&h.rgb_slot, L.rgb_slot);
+    if (!rc) rc = h.upload(&h.rgb_ratio, L.rgb_ratio);
+    if (!rc) rc = h.upload(&h.xnode_off, L.xnode_off);
+    if (!rc) rc = h.upload(&h.xnodes, L.xnodes);
+    if (!rc) rc = h.upload(&h.xentry_pair_off, L.xentry_pair_off);
+    if (!rc) rc = h.upload(&h.xpair_off, L.xpair_off);
+    if (!rc) rc = h.upload(&h.xpair_cell, L.xpair_cell);
+    if (!rc) rc = h.upload(&h.xpair_w, L.xpair_w);
+    if (!rc) rc = h.upload(&h.entry_pair_off, L.entry_pair_off);
+    if (!rc) rc = h.upload(&h.tile_pair_off, L.tile_pair_off);
+    if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
+    if (!rc) rc = h.upload(&h.pair_slot, L.pair_slot);
+    if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
+    if (rc) return rc;
+    h.pairs = true;
+    h.lds2 = lds2;
+    h.nrgc = nrgc;
+    h.nrgb = L.nrg;
+    h.max_xnodes = L.max_xnodes;
+    h.max_xpairs = L.max_xpairs;
+  }
   // node rho/vel hold the boundary-condition values (reference node order)
   std::vector<T> rho(h.N), vel(2 * (size_t)h.N);
   const T *mr = static_cast<const T *>(m.node_rho);
@@ -909,6 +1321,7 @@ int fvdbm_lattice_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, int
   h->scheme = p->scheme;
   h->block = block;
   h->graph_steps = gs;
+  h->two_step = (p->variant & 8) == 0;
   h->tau = p->tau;
   h->dt = p->delta_t;
   LHIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1011,7 +1424,7 @@ int fvdbm_lattice_info_get(void *handle, fvdbm_info *o) {
   o->bandwidth = h->bandwidth;
   o->precision = h->precision;
   o->scheme = h->scheme;
-  o->variant = 2 | 4 | 8;  // node kernel, no staging, one step per launch
+  o->variant = 4 | (h->pairs ? 0 : 8);  // no one-step staging; two-step tiles or not
   o->block = h->block;
   o->graph_steps = h->graph_steps;
   o->steps_done = h->steps;

@@ -51,7 +51,7 @@ def test_d2q13_cavity_matches_oracle(gpu, precision):
     st = oracle_state(oenv, nodes, p, 13)
     env = fv.Environment(cells, faces, nodes, precision=precision)
     env.init()
-    assert env.info()["variant"] == 14
+    assert env.info()["variant"] == 4  # two-step tiles (upwind), one step per launch otherwise
     W = np.asarray(oracle.W13_64, env.dtype)  # W in the working precision
     assert np.array_equal(env.cells.pdf, np.repeat(W[None], cells.size, 0))
     assert np.array_equal(env.faces.pdf, np.repeat(W[None], faces.size, 0))
