@@ -165,8 +165,10 @@ int fvdbm_synchronize(void *handle);
  * (quivers = 9, the same plain kernels, a cross-check of the tuned path).
  * Replaces Environment.init/step for an Environment built on D2Q13
  * (src/environment.py:38-65). Mesh descriptor and params as for
- * fvdbm_create (params.variant is ignored: one step per launch, boundary
- * nodes in their own kernel, no LDS staging); fields as for
+ * fvdbm_create (params.variant: bit 8 = one step per launch, other bits
+ * ignored; by default upwind handles advance two steps per launch where the
+ * tiles fit, and info reports variant 4, or 12 for one step per launch);
+ * fields as for
  * fvdbm_get_field / fvdbm_set_field with Q = quivers pdf components.
  * Errors through fvdbm_last_error(). No batched handles, residual or
  * warm-start entry points for this handle.
