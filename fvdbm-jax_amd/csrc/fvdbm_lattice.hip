@@ -62,8 +62,9 @@ __host__ __device__ constexpr int ky(int q) {
 template <typename T, int QN>
 struct Consts {
   T W[QN];
-  T C2, C4x2, C2x2, C6x2;  // C^2, 2 C^4, 2 C^2, 2 C^6
-  T invtau, dt;            // 1/tau rounded from double, delta_t rounded
+  T C2, C4x2, C2x2, C6x2;      // C^2, 2 C^4, 2 C^2, 2 C^6
+  T rC2, rC4x2, rC2x2, rC6x2;  // their rounded reciprocals (FP32 fast division)
+  T invtau, dt;                // 1/tau rounded from double, delta_t rounded
 };
 
 template <typename T, int QN>
@@ -82,6 +83,10 @@ static Consts<T, QN> make_consts(double tau, double dt) {
   k.C4x2 = T(2) * c4;
   k.C2x2 = T(2) * k.C2;
   k.C6x2 = T(2) * c6;
+  k.rC2 = T(1) / k.C2;
+  k.rC4x2 = T(1) / k.C4x2;
+  k.rC2x2 = T(1) / k.C2x2;
+  k.rC6x2 = T(1) / k.C6x2;
   k.invtau = T(1.0 / tau);
   k.dt = T(dt);
   return k;
@@ -104,6 +109,22 @@ __device__ __forceinline__ void macro(const T (&f)[QN], T &rho, T &ux, T &uy) {
   uy = my / r;
 }
 
+// velocities in pairs of opposites
+constexpr int kPos[6] = {1, 2, 5, 6, 9, 10};   // and their opposites:
+constexpr int kNeg[6] = {3, 4, 7, 8, 11, 12};  // ksi_neg = -ksi_pos
+
+// x / y for y one of the lattice constants (C^2, 2C^4, 2C^2, 2C^6) with
+// r = RN(1/y), in FP32: q0 = RN(x r), rem = x - q0 y (exact with one FMA),
+// q = RN(q0 + rem r). An exhaustive CPU check over every FP32 x whose
+// biased exponent lies in [25, 251] shows q equal to the IEEE quotient x / y
+// for each constant of D2Q9 and of D2Q13 (tests/fastdiv_check.c); for
+// x = +0 it returns +0, as the IEEE division does.
+__device__ __forceinline__ float fast_div(float x, float y, float r) {
+  const float q0 = x * r;
+  const float rem = __builtin_fmaf(-q0, y, x);
+  return __builtin_fmaf(rem, r, q0);
+}
+
 // (W rho) * ((((1 + cu/C^2) + cu^2/(2C^4)) - uu/(2C^2)) [D2Q13: + cu^3/(2C^6)
 // - 3 cu uu/(2C^4)]), cu = ksi.u and uu = u.u as 0-initialised sums,
 // cu^3 = cu*(cu*cu) (src/dynamics.py:70-74, 101-102).
@@ -112,23 +133,38 @@ __device__ __forceinline__ void macro(const T (&f)[QN], T &rho, T &ux, T &uy) {
 // the same cu^2/(2C^4): each quotient is formed once per pair of opposite
 // directions and each bracket is still summed in the reference's order for
 // every q. A literal per-q evaluation can differ only in the sign of a zero
-// cu, which leaves every bracket unchanged. q = 0 is evaluated literally.
-template <typename T, int QN>
-__device__ __forceinline__ void equilibrium(const Consts<T, QN> &k, T rho, T ux, T uy,
-                                            T (&out)[QN]) {
-  constexpr int kPos[6] = {1, 2, 5, 6, 9, 10};   // and their opposites:
-  constexpr int kNeg[6] = {3, 4, 7, 8, 11, 12};  // ksi_neg = -ksi_pos
+// cu, which leaves every bracket unchanged.
+// q = 0: cu0 = (0 + 0 ux) + 0 uy is +0 (or NaN), and dividing +0 or NaN by
+// a constant returns it unchanged, so its bracket uses cu0 in place of each
+// quotient: the literal values without the four divisions.
+//
+// FP32 with uu < 1/2 (false for inf and NaN) divides with fast_div, and the
+// brackets equal the IEEE ones: every dividend (cu, cu^2, cu^3, uu,
+// 3 cu uu; none is -0, the literal sums never are) is below 4. A dividend
+// of at least 2^-102 is divided exactly. A smaller one (zero included) has
+// a quotient below 2^-98 by either method, and it is added where the
+// partial bracket is at least 1/4 in magnitude, below half its ulp, so it
+// is absorbed either way: A, B (|cu| < 2^-51) and D (|u| < 2^-51) where the
+// partial sum is 1; E (|cu| < 2^-34, so A and B are absorbed) to
+// 1 - D >= 1/2 - 2^-24; G to that same bracket, or, when |cu| >= 2^-34, to
+// 1 (then uu < 2^-69, so |u| < 2^-34 and A, B, D, E are all below 2^-26).
+template <bool FAST, typename T, int QN>
+__device__ __forceinline__ void equilibrium_div(const Consts<T, QN> &k, T rho, T ux, T uy, T uu,
+                                                T (&out)[QN]) {
   constexpr int kPairs = QN == 13 ? 6 : 4;
-  T uu = T(0);
-  uu += ux * ux;
-  uu += uy * uy;
-  const T D = uu / k.C2x2;
+  auto div = [](T x, T y, T r) {
+    if constexpr (FAST)
+      return fast_div(x, y, r);
+    else
+      return x / y;
+  };
+  const T D = div(uu, k.C2x2, k.rC2x2);
   {
-    T cu = T(0);
-    cu += T(0) * ux;
-    cu += T(0) * uy;
-    T br = ((T(1) + cu / k.C2) + (cu * cu) / k.C4x2) - D;
-    if constexpr (QN == 13) br = (br + (cu * (cu * cu)) / k.C6x2) - ((T(3) * cu) * uu) / k.C4x2;
+    T c0 = T(0);
+    c0 += T(0) * ux;
+    c0 += T(0) * uy;
+    T br = ((T(1) + c0) + c0 * c0) - D;
+    if constexpr (QN == 13) br = (br + c0 * (c0 * c0)) - (T(3) * c0) * uu;
     out[0] = (k.W[0] * rho) * br;
   }
 #pragma unroll
@@ -137,16 +173,59 @@ __device__ __forceinline__ void equilibrium(const Consts<T, QN> &k, T rho, T ux,
     T cu = T(0);
     cu += T(kx(qp)) * ux;
     cu += T(ky(qp)) * uy;
-    const T A = cu / k.C2, B = (cu * cu) / k.C4x2;
+    const T A = div(cu, k.C2, k.rC2), B = div(cu * cu, k.C4x2, k.rC4x2);
     T bp = ((T(1) + A) + B) - D;
     T bn = ((T(1) + (-A)) + B) - D;
     if constexpr (QN == 13) {
-      const T E = (cu * (cu * cu)) / k.C6x2, G = ((T(3) * cu) * uu) / k.C4x2;
+      const T E = div(cu * (cu * cu), k.C6x2, k.rC6x2);
+      const T G = div((T(3) * cu) * uu, k.C4x2, k.rC4x2);
       bp = (bp + E) - G;
       bn = (bn + (-E)) - (-G);
     }
     out[qp] = (k.W[qp] * rho) * bp;
     out[qn] = (k.W[qn] * rho) * bn;
+  }
+}
+
+template <typename T, int QN>
+__device__ __forceinline__ void equilibrium(const Consts<T, QN> &k, T rho, T ux, T uy,
+                                            T (&out)[QN]) {
+  T uu = T(0);
+  uu += ux * ux;
+  uu += uy * uy;
+  if constexpr (std::is_same<T, float>::value) {
+    if (uu < 0.5f) {
+      equilibrium_div<true>(k, rho, ux, uy, uu, out);
+      return;
+    }
+  }
+  equilibrium_div<false>(k, rho, ux, uy, uu, out);
+}
+
+// Upwind ksi.n for q >= 1: opposite velocities give exact negatives and
+// ksi = (+-2, 0), (0, +-2) gives 2n exactly, so these equal the
+// 0-initialised dot products (src/containers.py:235) up to the sign of a
+// zero; a zero of either sign selects the same upwind side (phi >= 0) and
+// makes a zero flux, which an accumulator started at +0 absorbs alike.
+// q = 0 (ksi = 0) has a zero flux, and the callers skip it.
+template <typename T, int QN>
+__device__ __forceinline__ void upwind_phi(T nx, T ny, T (&phi)[QN]) {
+  const T p5 = nx + ny, p6 = ny - nx;
+  phi[0] = T(0);
+  phi[1] = nx;
+  phi[2] = ny;
+  phi[3] = -nx;
+  phi[4] = -ny;
+  phi[5] = p5;
+  phi[6] = p6;
+  phi[7] = -p5;
+  phi[8] = -p6;
+  if constexpr (QN == 13) {
+    const T a = nx + nx, b = ny + ny;
+    phi[9] = a;
+    phi[10] = b;
+    phi[11] = -a;
+    phi[12] = -b;
   }
 }
 
@@ -337,16 +416,21 @@ __global__ void __launch_bounds__(1024) k_lat_step(Args<T> a, Consts<T, QN> k) {
       la = a.lwa[at];
       den2 = a.lwden2[at];
     }
+    if constexpr (SCHEME == 0) {
+      T phi[QN];
+      upwind_phi<T, QN>(nx, ny, phi);
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      T phi = T(0);
-      phi += T(kx(q)) * nx;
-      phi += T(ky(q)) * ny;
-      if constexpr (SCHEME == 0) {
-        const bool take_slot0 = phi >= T(0);
+      for (int q = 1; q < QN; ++q) {
+        const bool take_slot0 = phi[q] >= T(0);
         const T up = (take_slot0 != slot1) ? f[q] : o[q];
-        acc[q] += (up * phi) * sL;
-      } else {
+        acc[q] += (up * phi[q]) * sL;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        T phi = T(0);
+        phi += T(kx(q)) * nx;
+        phi += T(ky(q)) * ny;
         const T fu = slot1 ? o[q] : f[q];
         const T fd = slot1 ? f[q] : o[q];
         const T interp = la - (phi * k.dt) / den2;
@@ -509,14 +593,13 @@ __device__ __forceinline__ void lat_update_staged(const Consts<T, QN> &k, const 
         o[q] = fbar + (fbar - f[q]) * ratio;
       }
     }
+    T phi[QN];
+    upwind_phi<T, QN>(nx[kf], ny[kf], phi);
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      T phi = T(0);
-      phi += T(kx(q)) * nx[kf];
-      phi += T(ky(q)) * ny[kf];
-      const bool take_slot0 = phi >= T(0);
+    for (int q = 1; q < QN; ++q) {
+      const bool take_slot0 = phi[q] >= T(0);
       const T up = (take_slot0 != slot1) ? f[q] : o[q];
-      acc[q] += (up * phi) * sL[kf];
+      acc[q] += (up * phi[q]) * sL[kf];
     }
   }
   T rho, ux, uy, eq[QN];

@@ -119,14 +119,18 @@ __device__ __forceinline__ void macro(const T (&f)[Q], T &rho, T &ux, T &uy) {
 // and four B quotients are distinct; each bracket is still formed per q.
 // x / y for one of the three lattice constants y with r = RN(1/y).
 // FP32: q0 = RN(x r), rem = x - q0 y (exact with one FMA), q = RN(q0 + rem r).
-// An exhaustive CPU check over all 2^32 inputs shows q equals the IEEE
+// An exhaustive CPU check (tests/fastdiv_check.c) shows q equal to the IEEE
 // quotient x / y for every x whose biased exponent lies in [25, 251], for
-// each of the three constants; the kernels use it when every dividend of an
-// equilibrium lies in [40, 251] (remainders stay normal) and the IEEE
-// division otherwise (zero, tiny, huge, inf, nan).
-__device__ __forceinline__ bool fast_div_ok(float x) {
-  return ((__float_as_uint(x) >> 23) & 0xffu) - 40u <= 211u;
-}
+// each of the three constants, and q = +0 for x = +0.
+//
+// The equilibrium uses it when uu < 1/2 (false for inf and NaN), and the
+// IEEE division otherwise. With uu < 1/2 every dividend (ux, uy, c5, c6,
+// their squares, uu) is below 4. One of at least 2^-102 is divided exactly;
+// a smaller one (zero included) has a quotient below 2^-98 by either
+// method, and it is added where the partial bracket is 1 (A: |ksi.u| <
+// 2^-102; B: |ksi.u| < 2^-51, so A is absorbed; D: |u| < 2^-51, so every A
+// and B is), below half an ulp of 1, so the bracket is the IEEE one either
+// way. A zero of either sign in A leaves the bracket unchanged.
 __device__ __forceinline__ float fast_div(float x, float y, float r) {
   const float q0 = x * r;
   const float rem = __builtin_fmaf(-q0, y, x);
@@ -149,13 +153,7 @@ template <>
 __device__ __forceinline__ EqTerms<float> eq_terms<float>(const Lattice<float> &k, float ux,
                                                           float uy, float uu, float c5, float c6) {
   const float s1 = ux * ux, s2 = uy * uy, s5 = c5 * c5, s6 = c6 * c6;
-  // biased exponents of ux, uy, c5, c6 in [84, 188] put them and the five
-  // squares and sums derived from them (uu >= ux*ux) inside [40, 251]
-  auto ex = [](float v) { return (__float_as_uint(v) >> 23) & 0xffu; };
-  const uint32_t emin = min(min(ex(ux), ex(uy)), min(ex(c5), ex(c6)));
-  const uint32_t emax = max(max(ex(ux), ex(uy)), max(ex(c5), ex(c6)));
-  const bool ok = emin >= 84u && emax <= 188u;
-  if (ok) {
+  if (uu < 0.5f) {
     return {fast_div(uu, k.C2x2, k.rC2x2), fast_div(ux, k.C2, k.rC2),
             fast_div(uy, k.C2, k.rC2),     fast_div(c5, k.C2, k.rC2),
             fast_div(c6, k.C2, k.rC2),     fast_div(s1, k.C4x2, k.rC4x2),

@@ -1,8 +1,10 @@
-"""The step kernel divides by the three lattice constants with an
-FMA-corrected reciprocal multiply on dividends whose biased exponent lies in
-[40, 251]. This checks, exhaustively on the CPU, that the method returns the
-IEEE quotient bit for bit for every such FP32 input (the method is odd in x,
-so the positive half covers both signs)."""
+"""The step kernels divide by the lattice constants (D2Q9: C^2, 2C^4, 2C^2;
+D2Q13: C^2, 2C^4, 2C^2, 2C^6) with an FMA-corrected reciprocal multiply on
+dividends whose biased exponent lies in [25, 251], and on +0 (smaller
+dividends are absorbed in the equilibrium's brackets; the argument is in
+the kernels). This checks, exhaustively on the CPU, that the method returns
+the IEEE quotient bit for bit for every such FP32 input (the method is odd
+in x, so the positive half covers both signs)."""
 import os
 import shutil
 import subprocess
@@ -19,9 +21,9 @@ def test_fast_division_is_exact_on_kernel_range():
         exe = os.path.join(d, "fastdiv_check")
         subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-mfma",
                         os.path.join(HERE, "fastdiv_check.c"), "-o", exe, "-lm"], check=True)
-        out = subprocess.run([exe, "40", "251"], check=True, capture_output=True, text=True,
+        out = subprocess.run([exe, "25", "251"], check=True, capture_output=True, text=True,
                              env=dict(os.environ, OMP_NUM_THREADS=str(min(8, os.cpu_count() or 1))))
     rows = [line.split() for line in out.stdout.strip().splitlines()]
-    assert len(rows) == 3
+    assert len(rows) == 7
     for y, bad in rows:
         assert int(bad) == 0, f"division by {y}: {bad} mismatches"
