@@ -186,3 +186,23 @@ def test_generated_meshes_are_valid():
     assert np.all(on_rect | on_disc)
     st = m.verify_stencil_geometry(quiet=True)
     assert st["max_distance_error"] < 1e-9
+
+
+def test_import_meshpy_equals_from_raw():
+    """Mesher.import_meshpy (src/mesher.py:48-61) takes the attributes of a
+    meshpy.triangle.MeshInfo (points, elements, faces, point_markers; meshpy
+    itself is not needed) and orients the cells CCW: the result equals the
+    from_raw path on the same mesh, also when half the cells come clockwise."""
+    from types import SimpleNamespace
+    raw = _scrambled(meshgen.jittered_delaunay(1.0, 0.5, [(0.4, 0.25, 0.1)], 0.05, seed=3), 7)
+    info = SimpleNamespace(points=raw.points.tolist(), elements=raw.cells.tolist(),
+                           faces=raw.faces.tolist(), point_markers=raw.point_markers.tolist())
+    m = fv.Mesher()
+    m.import_meshpy(info)
+    assert m.points.dtype == np.float64 and m.cells.dtype == np.int32
+    m.calc_mesh_properties()
+    ref = build_mesher(raw)
+    for key in INT_KEYS:
+        assert np.array_equal(getattr(m, key), getattr(ref, key)), key
+    for key in FLOAT_KEYS:
+        assert np.array_equal(getattr(m, key), getattr(ref, key)), key
