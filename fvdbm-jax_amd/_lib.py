@@ -47,7 +47,8 @@ class Info(ct.Structure):
         "n_cells", "n_faces", "n_nodes", "k_max", "n_cells_padded",
         "n_boundary_cells", "n_boundary_faces", "n_active_nodes", "bandwidth",
         "precision", "scheme", "variant", "block", "graph_steps")] + [
-        ("steps_done", ct.c_int64), ("device_bytes", ct.c_int64), ("n_cases", ct.c_int32)]
+        ("steps_done", ct.c_int64), ("device_bytes", ct.c_int64), ("n_cases", ct.c_int32),
+        ("launch_classes", ct.c_int32)]
 
 
 class BatchDesc(ct.Structure):

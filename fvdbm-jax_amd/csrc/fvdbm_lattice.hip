@@ -1513,6 +1513,7 @@ int fvdbm_lattice_info_get(void *handle, fvdbm_info *o) {
   o->steps_done = h->steps;
   o->device_bytes = h->device_bytes;
   o->n_cases = 1;
+  o->launch_classes = h->pairs ? 1 : 0;
   return FVDBM_OK;
 }
 

@@ -365,7 +365,7 @@ struct PairArgs {
   const int32_t *__restrict__ pair_slot;   // stage slot of each own-table pair
   int nrg_cells;                           // ring cells with face records (all tiles)
   int nrgb;
-  int max_xnodes, max_xpairs, max_ring;
+  int max_xnodes, max_xpairs;              // LDS partition: table rows, pair records
 };
 
 // The 9 pdfs of cell c from a q-major [9][Cp] buffer.
@@ -1295,7 +1295,16 @@ struct Handle {
   // upwind handles whose rings and LDS fit
   bool pairs = false;
   size_t lds2 = 0;
-  int nrg_cells = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0, max_ring = 0;
+  // LDS classes of the two-step launch: with npair_class = 2, tiles whose
+  // LDS fits pair_lds[0] run in a first launch sized for it (so LDS does not
+  // limit its occupancy) and the others in a second one; 0 = one launch over
+  // block_tile with lds2
+  int npair_class = 0;
+  void *pair_bt[2] = {nullptr, nullptr};
+  int pair_grid[2] = {0, 0};
+  size_t pair_lds[2] = {0, 0};
+  int pair_xnodes[2] = {0, 0}, pair_xpairs[2] = {0, 0};
+  int nrg_cells = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0;
   void *ring_off = nullptr, *ring_cells = nullptr, *ring_n1 = nullptr, *rg_off = nullptr,
        *rg_geo = nullptr, *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr,
        *xnodes = nullptr, *xentry_pair_off = nullptr, *xpair_off = nullptr,
@@ -1493,10 +1502,22 @@ static int launch_pair_typed(Handle &h, int src) {
   p.nrgb = h.nrgb;
   p.max_xnodes = h.max_xnodes;
   p.max_xpairs = h.max_xpairs;
-  p.max_ring = h.max_ring;
-  hipLaunchKernelGGL((k_step2<T>), dim3(grid_for(h, a.xcd_remap)), dim3(h.block), h.lds2, h.stream,
-                     a, p);
-  HIP_TRY(hipGetLastError());
+  if (h.npair_class == 0) {
+    hipLaunchKernelGGL((k_step2<T>), dim3(grid_for(h, a.xcd_remap)), dim3(h.block), h.lds2,
+                       h.stream, a, p);
+    HIP_TRY(hipGetLastError());
+    return FVDBM_OK;
+  }
+  // the classes partition the tiles, and a tile reads only fsrc and writes
+  // only its own cells of fdst: two launches are one two-step launch
+  for (int k = 0; k < h.npair_class; ++k) {
+    a.block_tile = static_cast<const int32_t *>(h.pair_bt[k]);
+    p.max_xnodes = h.pair_xnodes[k];
+    p.max_xpairs = h.pair_xpairs[k];
+    hipLaunchKernelGGL((k_step2<T>), dim3(h.pair_grid[k]), dim3(h.block), h.pair_lds[k],
+                       h.stream, a, p);
+    HIP_TRY(hipGetLastError());
+  }
   return FVDBM_OK;
 }
 
@@ -1722,6 +1743,98 @@ static int run_steps(Handle &h, int64_t n, bool resid_last) {
   return FVDBM_OK;
 }
 
+// LDS of a two-step launch over a set of tiles: the kernel partitions it
+// with the set's largest node table (rows), largest pair count (extended or
+// own table) and largest ring: [9][rows] table, pair records rounded to 8
+// words, then the stage of block + ring cell records.
+struct PairLds {
+  int rows = 0, pairs = 0, ring = 0;
+  template <typename T>
+  void add(const HostLayout<T> &L, int t) {
+    rows = std::max(rows, L.xnode_off[t + 1] - L.xnode_off[t]);
+    pairs = std::max(pairs, std::max(L.xpair_off[t + 1] - L.xpair_off[t],
+                                     L.tile_pair_off[t + 1] - L.tile_pair_off[t]));
+    ring = std::max(ring, L.ring_off[t + 1] - L.ring_off[t]);
+  }
+  size_t bytes(int block, size_t es) const {
+    const size_t pre = ((size_t)Q * rows + kPairW * (size_t)pairs + 7) & ~(size_t)7;
+    return (pre + (size_t)9 * (block + ring)) * es;
+  }
+};
+
+// LDS classes of the two-step launch. One launch sizes its LDS for the
+// largest tile; where that LDS, not the registers, limits the resident
+// blocks per CU (boundary-heavy meshes: a few tiles with large node tables),
+// the tiles that fit the register-limited share of the CU's LDS run in a
+// launch of their own with that smaller size, the rest in a second launch.
+// Each class keeps the XCD-aware order (boundary tiles first per range).
+// Without occupancy information the single launch stays.
+template <typename T>
+static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
+  int dev = 0, lds_cu = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_step2<T>),
+                                                   h.block, 0) != hipSuccess ||
+      nb <= 0 || lds_cu <= 0)
+    return FVDBM_OK;
+  const size_t es = sizeof(T);
+  const size_t budget = ((size_t)lds_cu / (size_t)nb) & ~(size_t)1023;
+  if (h.lds2 <= budget) return FVDBM_OK;  // LDS does not limit the occupancy
+  std::vector<size_t> need(L.ntiles);
+  for (int t = 0; t < L.ntiles; ++t) {
+    PairLds one;
+    one.add(L, t);
+    need[t] = one.bytes(h.block, es);
+  }
+  // the light class: tiles up to a threshold, lowered until the class's
+  // LDS (from its component maxima) fits the budget
+  size_t thr = budget;
+  PairLds cls[2];
+  int n_light = 0;
+  for (;;) {
+    cls[0] = PairLds();
+    cls[1] = PairLds();
+    n_light = 0;
+    for (int t = 0; t < L.ntiles; ++t) {
+      const int k = need[t] <= thr ? 0 : 1;
+      cls[k].add(L, t);
+      n_light += k == 0;
+    }
+    if (cls[0].bytes(h.block, es) <= budget || thr < 1024) break;
+    thr -= 512;
+  }
+  if (n_light == 0 || n_light == L.ntiles || cls[0].bytes(h.block, es) > budget ||
+      cls[1].bytes(h.block, es) > 65536)
+    return FVDBM_OK;
+  const int per = (L.ntiles + 7) / 8;
+  for (int k = 0; k < 2; ++k) {
+    std::vector<int32_t> order[8];
+    size_t longest = 0;
+    for (int x = 0; x < 8; ++x) {
+      const int lo = std::min(x * per, L.ntiles), hi = std::min((x + 1) * per, L.ntiles);
+      for (int pass = 0; pass < 2; ++pass)  // tiles with boundary nodes first
+        for (int t = lo; t < hi; ++t) {
+          const bool nodes = L.tile_node_off[t + 1] > L.tile_node_off[t];
+          if (nodes == (pass == 0) && (need[t] <= thr) == (k == 0)) order[x].push_back(t);
+        }
+      longest = std::max(longest, order[x].size());
+    }
+    std::vector<int32_t> bt(8 * longest, -1);
+    for (int x = 0; x < 8; ++x)
+      for (size_t i = 0; i < order[x].size(); ++i) bt[i * 8 + x] = order[x][i];
+    const int rc = h.upload(&h.pair_bt[k], bt);
+    if (rc) return rc;
+    h.pair_grid[k] = (int)bt.size();
+    h.pair_lds[k] = cls[k].bytes(h.block, es);
+    h.pair_xnodes[k] = cls[k].rows;
+    h.pair_xpairs[k] = cls[k].pairs;
+  }
+  h.npair_class = 2;
+  return FVDBM_OK;
+}
+
 template <typename T>
 static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell_case = nullptr,
                         int ncases = 1) {
@@ -1934,8 +2047,8 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
         h.nrgb = L.nrg;
         h.max_xnodes = L.max_xnodes;
         h.max_xpairs = L.max_xpairs;
-        h.max_ring = L.max_ring;
       }
+      if (!rc && !(h.variant & 1)) rc = build_pair_classes<T>(h, L);
     }
   }
   if (!rc && !h.pairs) h.variant |= 8;  // one step per launch (two-step tiles off or unavailable)
@@ -2518,6 +2631,7 @@ int fvdbm_info_get(void *handle, fvdbm_info *o) {
   o->steps_done = h->steps;
   o->device_bytes = h->device_bytes;
   o->n_cases = h->ncases;
+  o->launch_classes = h->pairs ? std::max(h->npair_class, 1) : 0;
   return FVDBM_OK;
 }
 

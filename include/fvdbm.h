@@ -110,6 +110,8 @@ typedef struct fvdbm_info {
   int64_t steps_done;
   int64_t device_bytes;     /* device memory owned by the handle */
   int32_t n_cases;          /* 1, or the case count of a batched handle */
+  int32_t launch_classes;   /* launches per two-step launch: 1, or 2 when tiles are split by
+                               their LDS size; 0 without two-step tiles */
 } fvdbm_info;
 
 /* Batched solves (no counterpart in the reference, which runs one mesh per

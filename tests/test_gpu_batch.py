@@ -48,6 +48,27 @@ def test_batch_of_one_is_single(gpu):
     assert np.array_equal(b.get_pdf(), single)
 
 
+def test_lds_classes_equal_one_launch(gpu):
+    """Where the largest tile's LDS, not the registers, limits the resident
+    blocks per CU, a two-step launch is split into two launches by tile LDS
+    size. The result equals one launch over all tiles (variant bit 1: plain
+    tile order, never split), at every block size; small blocks on these
+    boundary-heavy meshes take the split."""
+    cs = [(c, f, n) for _, _, c, f, n in _cases(3)]
+    classes = []
+    for block in (64, 128, 256):
+        b = batch.BatchSolver(cs, block=block)
+        ref = batch.BatchSolver(cs, block=block, variant=1)
+        classes.append(b.info()["launch_classes"])
+        assert ref.info()["launch_classes"] == 1
+        b.step(37)
+        ref.step(37)
+        assert np.array_equal(b.get_pdf(), ref.get_pdf()), block
+        b.close()
+        ref.close()
+    assert 2 in classes, classes
+
+
 @pytest.mark.parametrize("variant", [0, 2, 8])
 def test_run_residuals_and_freezing(gpu, variant):
     """variant 0 runs two steps per launch (two-step tiles), 8 one step per
