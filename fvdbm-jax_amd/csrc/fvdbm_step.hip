@@ -1777,8 +1777,10 @@ static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
           hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_step2<T>),
                                                    h.block, 0) != hipSuccess ||
-      nb <= 0 || lds_cu <= 0)
+      nb <= 0 || lds_cu <= 0) {
+    (void)hipGetLastError();  // leave no error for the next launch check to report
     return FVDBM_OK;
+  }
   const size_t es = sizeof(T);
   const size_t budget = ((size_t)lds_cu / (size_t)nb) & ~(size_t)1023;
   if (h.lds2 <= budget) return FVDBM_OK;  // LDS does not limit the occupancy
