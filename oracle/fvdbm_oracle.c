@@ -71,6 +71,9 @@ enum {
   MUT_IDW_UNIFORM = 6,      /* IDW weights replaced by 1 */
   MUT_IDW_UNNORMALISED = 7, /* IDW sum not divided by sum of weights */
   MUT_NO_COLLISION = 8,     /* BGK term dropped */
+  MUT_LW_D1_FOR_D0 = 9,     /* Lax-Wendroff: d1 / (d0 + d1) in place of d0 / (d0 + d1) */
+  MUT_LW_DT_SIGN = 10,      /* Lax-Wendroff: + (phi dt) / (2 (d0 + d1)) */
+  MUT_LW_SWAP_UD = 11,      /* Lax-Wendroff: upwind and downwind slots exchanged */
 };
 
 typedef struct {

@@ -21,7 +21,7 @@ _SO = os.path.join(_HERE, "_build", "libfvdbm_oracle.so")
 MUTATIONS = {
     "none": 0, "downwind": 1, "no_ghost": 2, "no_sign": 3, "transpose_ksi": 4,
     "drop_quadratic": 5, "idw_uniform": 6, "idw_unnormalised": 7,
-    "no_collision": 8,
+    "no_collision": 8, "lw_d1_for_d0": 9, "lw_dt_sign": 10, "lw_swap_ud": 11,
 }
 
 # src/dynamics.py:54-63

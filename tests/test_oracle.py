@@ -4,8 +4,10 @@ The reference cannot run here (jax is absent) and its own tests hold no
 golden vectors, so the oracle is pinned by what the mathematics fixes:
 moment identities of the equilibrium, IDW of a constant field, the uniform
 fixed point, global mass balance, mirror symmetry, and a brute-force step on
-a two-triangle mesh written from the textbook cell-centred form (outflow
-uses the cell's own pdf, inflow the neighbour's). ``test_planted_mistakes``
+a two-triangle mesh and on a fan of irregular triangles, written from the
+textbook cell-centred form (upwind: outflow uses the cell's own pdf,
+inflow the neighbour's; Lax-Wendroff: the second-order face value from
+both sides' centre distances). ``test_planted_mistakes``
 shows that each plausible mistake planted in the oracle fails at least one
 of these checks.
 """
@@ -97,24 +99,32 @@ def check_mass_balance(mutation):
     return bool(abs(lhs + dyn.delta_t * out) < 1e-12 * np.abs(f0).sum())
 
 
-def _two_cells():
+def _two_cells(scheme="upwind"):
     pts = np.array([[0.0, 0.0], [1.0, 0.1], [1.1, 1.0], [-0.1, 0.9]])
     raw = meshgen.RawMesh(pts, np.array([[0, 1, 2], [0, 2, 3]], np.int32),
                           np.array([[0, 1], [1, 2], [2, 0], [2, 3], [3, 0]], np.int32),
                           np.array([1, 1, 2, 2], np.int32))
     m = build_mesher(raw)
     dyn = fv.D2Q9(tau=0.7, delta_t=0.05)
-    cells, faces, nodes = m.to_env(dyn)
+    cells, faces, nodes = m.to_env(dyn, scheme)
     nodes = m.set_vel_node(nodes, 1, np.array([0.03, -0.02]))
     nodes = m.set_rho_node(nodes, 2, 1.02)
     return m, dyn, cells, faces, nodes
 
 
-def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt):
+def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt, scheme="upwind"):
     """One step written cell-centred from the method's definition:
     node pdfs from inverse-distance weights, ghost = 2 f_face - f_cell,
-    outflow through an edge carries the cell's own pdf, inflow the other
-    side's, and f += dt ((feq - f)/tau - sum_edges L (c.n_out) f_upwind)."""
+    and f += dt ((feq - f)/tau - sum_edges L (c.n_out) f_face) with
+    f_face, per direction c, from the cell's own side (U, distance d_U
+    from its centroid to the edge line) and the other side (D, d_D; a
+    ghost sits at the cell's own distance):
+      upwind:       f_U where c.n_out >= 0, else f_D;
+      Lax-Wendroff: f_U + (f_D - f_U) (d_U/(d_U+d_D) - (c.n_out) dt/(2 (d_U+d_D))),
+    the textbook second-order face value for speed c.n_out over the
+    centre distance d_U + d_D. It does not depend on which side is
+    called U: exchanging U and D negates c.n and maps the bracket to
+    1 minus itself, so it is the same value seen from either cell."""
     C = len(tris)
     cen = points[tris].mean(axis=1)
     rho = f.sum(1)
@@ -147,30 +157,84 @@ def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt):
             if np.dot(n, cen[c] - (points[a] + points[b]) / 2) > 0:
                 n = -n
             other = [o for o in owners[frozenset((a, b))] if o != c]
+            mid = (points[a] + points[b]) / 2
+            dU = abs(np.dot(n, cen[c] - mid))
             if other:
                 fo = f[other[0]]
+                dD = abs(np.dot(n, cen[other[0]] - mid))
             else:
                 fo = 2 * (npdf[a] + npdf[b]) / 2 - f[c]
+                dD = dU
             cn = KSI @ n
-            total += L * cn * np.where(cn >= 0, f[c], fo)
+            if scheme == "upwind":
+                face = np.where(cn >= 0, f[c], fo)
+            else:
+                face = f[c] + (fo - f[c]) * (dU / (dU + dD) - cn * dt / (2 * (dU + dD)))
+            total += L * cn * face
         new[c] = f[c] + dt * ((feq[c] - f[c]) / tau - total)
     return new, npdf
 
 
-def check_brute_force(mutation):
-    m, dyn, cells, faces, nodes = _two_cells()
+def check_brute_force(mutation, scheme="upwind"):
+    m, dyn, cells, faces, nodes = _two_cells(scheme)
     env = oracle_env(cells, faces, nodes, "f64")
     st = oracle_state(env, nodes, "f64")
     rng = np.random.default_rng(4)
     f = feq_textbook(np.array([1.0, 1.03]), np.array([[0.04, 0.01], [-0.02, 0.03]])) * \
         (1 + 0.05 * rng.standard_normal((2, 9)))
     st["cell_pdf"] = f.copy()
-    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f64", mutation)
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f64", mutation, scheme=scheme)
     new, npdf = brute_force_step(m.points, m.cells, f, nodes.type.reshape(-1),
-                                 nodes.rho_f64.reshape(-1), nodes.vel_f64, dyn.tau, dyn.delta_t)
+                                 nodes.rho_f64.reshape(-1), nodes.vel_f64, dyn.tau, dyn.delta_t,
+                                 scheme)
     active = nodes.type.reshape(-1) > 0
     return bool(np.allclose(st["cell_pdf"], new, rtol=0, atol=1e-14)
                 and np.allclose(st["node_pdf"][active], npdf[active], rtol=0, atol=1e-14))
+
+
+def _fan(n=7, seed=5):
+    """n irregular triangles around one interior point, with a velocity and
+    a density marker on the rim: interior faces of unequal centre distances
+    on both sides."""
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rad = rng.uniform(0.7, 1.3, n)
+    pts = np.vstack([[[0.05, -0.03]], np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)])
+    tris = np.array([[0, 1 + i, 1 + (i + 1) % n] for i in range(n)], np.int32)
+    edges = [[0, 1 + i] for i in range(n)] + [[1 + i, 1 + (i + 1) % n] for i in range(n)]
+    markers = np.array([0] + [1 if i < n // 2 else 2 for i in range(n)], np.int32)
+    return meshgen.RawMesh(pts, tris, np.array(edges, np.int32), markers)
+
+
+def check_brute_force_fan(mutation, scheme):
+    m = build_mesher(_fan())
+    dyn = fv.D2Q9(tau=0.65, delta_t=0.04)
+    cells, faces, nodes = m.to_env(dyn, scheme)
+    nodes = m.set_vel_node(nodes, 1, np.array([0.02, 0.035]))
+    nodes = m.set_rho_node(nodes, 2, 0.98)
+    env = oracle_env(cells, faces, nodes, "f64")
+    st = oracle_state(env, nodes, "f64")
+    C = len(m.cells)
+    rng = np.random.default_rng(6)
+    f = feq_textbook(1 + 0.02 * rng.standard_normal(C), 0.04 * rng.standard_normal((C, 2))) * \
+        (1 + 0.05 * rng.standard_normal((C, 9)))
+    st["cell_pdf"] = f.copy()
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f64", mutation, scheme=scheme)
+    new, _ = brute_force_step(m.points, m.cells, f, nodes.type.reshape(-1),
+                              nodes.rho_f64.reshape(-1), nodes.vel_f64, dyn.tau, dyn.delta_t,
+                              scheme)
+    return bool(np.allclose(st["cell_pdf"], new, rtol=0, atol=1e-14))
+
+
+def check_lw_fixed_point(mutation):
+    """A uniform equilibrium with zero-velocity walls stays put under the
+    Lax-Wendroff flux (constant fields interpolate to themselves)."""
+    m, dyn, cells, faces, nodes = ldc_case(u_lid=0.0, scheme="lax_wendroff")
+    env = oracle_env(cells, faces, nodes, "f64")
+    st = oracle_state(env, nodes, "f64")
+    f0 = st["cell_pdf"].copy()
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 100, "f64", mutation, scheme="lax_wendroff")
+    return bool(np.max(np.abs(st["cell_pdf"] - f0)) < 1e-13)
 
 
 MIRROR_Q = [0, 3, 2, 1, 4, 6, 5, 8, 7]
@@ -204,7 +268,13 @@ CHECKS = {
     "mass_balance": check_mass_balance,
     "brute_force": check_brute_force,
     "mirror": check_mirror,
+    "brute_force_fan": lambda mut: check_brute_force_fan(mut, "upwind"),
+    "lw_brute_force": lambda mut: check_brute_force(mut, "lax_wendroff"),
+    "lw_brute_force_fan": lambda mut: check_brute_force_fan(mut, "lax_wendroff"),
+    "lw_fixed_point": check_lw_fixed_point,
 }
+
+LW_MUTATIONS = [k for k in oracle.MUTATIONS if k.startswith("lw_")]
 
 
 @pytest.mark.parametrize("name", list(CHECKS))
@@ -216,6 +286,17 @@ def test_faithful_oracle_passes(name):
 def test_planted_mistakes(mutation):
     failed = [name for name, chk in CHECKS.items() if not chk(mutation)]
     assert failed, f"planted mistake {mutation!r} passes every oracle check"
+
+
+@pytest.mark.parametrize("mutation", LW_MUTATIONS)
+def test_lax_wendroff_mistakes_fail_the_brute_force(mutation):
+    """Each Lax-Wendroff mistake (reference src/containers.py:266-277: d1
+    for d0, the sign of the dt term, U and D exchanged) fails the
+    brute-force Lax-Wendroff step on both meshes, while the upwind checks
+    (which never take that branch) still pass."""
+    assert not check_brute_force(mutation, "lax_wendroff")
+    assert not check_brute_force_fan(mutation, "lax_wendroff")
+    assert check_brute_force(mutation, "upwind")
 
 
 def test_fp32_matches_fp64_on_cavity():
@@ -236,17 +317,6 @@ def test_fp32_matches_fp64_on_cavity():
 def test_eq_of_rest_state_is_w():
     out = oracle.calc_eq(np.ones(3), np.zeros((3, 2)), "f32")
     assert np.array_equal(out, np.repeat(np.float32(W)[None], 3, 0))
-
-
-def test_lax_wendroff_uniform_flow_is_preserved():
-    """A uniform equilibrium with matching boundary velocity is a fixed point
-    of the Lax-Wendroff flux too (constant fields interpolate to themselves)."""
-    m, dyn, cells, faces, nodes = ldc_case(u_lid=0.0, scheme="lax_wendroff")
-    env = oracle_env(cells, faces, nodes, "f64")
-    st = oracle_state(env, nodes, "f64")
-    f0 = st["cell_pdf"].copy()
-    oracle.run(env, st, dyn.tau, dyn.delta_t, 100, "f64", scheme="lax_wendroff")
-    assert np.max(np.abs(st["cell_pdf"] - f0)) < 1e-13
 
 
 def test_oracle_reproduces_committed_fixtures():
