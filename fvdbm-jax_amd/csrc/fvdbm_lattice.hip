@@ -823,6 +823,12 @@ struct Handle {
        *pair_slot = nullptr, *pair_w = nullptr;
   hipGraph_t graph2[2] = {nullptr, nullptr};
   hipGraphExec_t gexec2[2] = {nullptr, nullptr};
+  // two-step handles: after a run that ended in a two-step launch
+  // (prev_two_back), f[cur ^ 1] holds the pdf two steps back; the pdf at the
+  // start of the last step is one step from it, computed into fmid only
+  // when an observable is read (mid_ready)
+  void *fmid = nullptr;
+  bool prev_two_back = false, mid_ready = false;
   int cur = 0;
   bool has_prev = false, observed = false;
   int64_t steps = 0, device_bytes = 0;
@@ -904,8 +910,9 @@ static inline dim3 grid1(int n) { return dim3((unsigned)std::max(1, (n + 255) / 
 
 // one step reading f[src]: the active nodes, then the cells
 template <typename T, int QN>
-static int launch_step(Handle &h, int src) {
-  const Args<T> a = make_args<T>(h, src);
+static int launch_step(Handle &h, int src, void *dst = nullptr) {
+  Args<T> a = make_args<T>(h, src);
+  if (dst) a.fdst = static_cast<T *>(dst);
   const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
   const dim3 grid((unsigned)h.grid);
   const size_t lds = (size_t)QN * h.max_tile_nodes * sizeof(T);
@@ -1020,10 +1027,17 @@ static int ensure_graph(Handle &h, int parity) {
 
 static int run_steps(Handle &h, int64_t n) {
   if (n <= 0) return FVDBM_OK;
-  if (h.pairs && n >= 3) {
-    // pairs of steps for all but the last one or two, so the last step is
-    // a single launch and f[cur ^ 1] keeps the pdf at its start
-    int64_t rest = n - 1;
+  if (h.pairs && n >= 2) {
+    // two-step launches only; an odd count starts with one single step.
+    // f[cur ^ 1] then holds the pdf two steps back (see prev_two_back)
+    if (n & 1) {
+      int rc = step_any(h, h.cur);
+      if (rc) return rc;
+      h.cur ^= 1;
+      ++h.steps;
+      --n;
+    }
+    int64_t rest = n - 2;
     if (h.graph_steps >= 2) {
       while (rest >= h.graph_steps) {
         int rc = ensure_graph2(h, h.cur);
@@ -1034,13 +1048,17 @@ static int run_steps(Handle &h, int64_t n) {
         h.steps += h.graph_steps;
       }
     }
-    for (; rest >= 2; rest -= 2) {
+    for (; rest >= 0; rest -= 2) {
       int rc = pair_any(h, h.cur);
       if (rc) return rc;
       h.cur ^= 1;
       h.steps += 2;
     }
-    n = rest + 1;  // the remaining single steps, the last included
+    h.prev_two_back = true;
+    h.mid_ready = false;
+    h.has_prev = true;
+    h.observed = false;
+    return FVDBM_OK;
   }
   if (h.graph_steps > 0) {
     while (n >= h.graph_steps) {
@@ -1057,6 +1075,7 @@ static int run_steps(Handle &h, int64_t n) {
     h.cur ^= 1;
     ++h.steps;
   }
+  h.prev_two_back = false;
   h.has_prev = true;
   h.observed = false;
   return FVDBM_OK;
@@ -1075,6 +1094,7 @@ static int init_uniform(Handle &h) {
   h.cur = 0;
   h.steps = 0;
   h.has_prev = false;
+  h.prev_two_back = false;
   h.observed = false;
   return FVDBM_OK;
 }
@@ -1191,6 +1211,7 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
     if (!rc) rc = h.upload(&h.pair_cell, L.pair_cell);
     if (!rc) rc = h.upload(&h.pair_slot, L.pair_slot);
     if (!rc) rc = h.upload(&h.pair_w, L.pair_w);
+    if (!rc) rc = h.alloc(&h.fmid, (size_t)QN * h.Cp * sizeof(T));
     if (rc) return rc;
     h.pairs = true;
     h.lds2 = lds2;
@@ -1213,6 +1234,19 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
   return init_uniform<T, QN>(h);
 }
 
+// After a run that ended in a two-step launch: the pdf at the start of the
+// last step, one step from f[cur ^ 1] (bitwise the two-step launch's
+// intermediate state), into fmid. Done before anything the step reads
+// (boundary-node values) changes, and at most once per run.
+template <typename T, int QN>
+static int materialize_mid(Handle &h) {
+  if (!h.has_prev || !h.prev_two_back || h.mid_ready) return FVDBM_OK;
+  const int rc = launch_step<T, QN>(h, h.cur ^ 1, h.fmid);
+  if (rc) return rc;
+  h.mid_ready = true;
+  return FVDBM_OK;
+}
+
 // derived fields of the pdf at the start of the last step
 template <typename T, int QN>
 static int observe(Handle &h) {
@@ -1223,7 +1257,10 @@ static int observe(Handle &h) {
   if (!rc && !h.obs_feq) rc = h.alloc(&h.obs_feq, (size_t)QN * h.Cp * sizeof(T));
   if (!rc && !h.obs_flux) rc = h.alloc(&h.obs_flux, (size_t)QN * h.F * sizeof(T));
   if (rc) return rc;
-  const Args<T> a = make_args<T>(h, h.cur ^ 1);
+  rc = materialize_mid<T, QN>(h);
+  if (rc) return rc;
+  Args<T> a = make_args<T>(h, h.cur ^ 1);
+  if (h.prev_two_back) a.fsrc = static_cast<const T *>(h.fmid);
   const Consts<T, QN> k = make_consts<T, QN>(h.tau, h.dt);
   if (h.nan > 0)  // node values at the start of the last step; 16 nodes per block
     hipLaunchKernelGGL((k_lat_nodes<T, QN>), dim3((unsigned)((h.nan + 15) / 16)), dim3(256), 0,
@@ -1328,6 +1365,11 @@ template <typename T, int QN>
 static int set_typed(Handle &h, int field, const void *src) {
   const T *in = static_cast<const T *>(src);
   const size_t es = sizeof(T);
+  if (field == FVDBM_NODE_PDF || field == FVDBM_NODE_VEL || field == FVDBM_NODE_RHO) {
+    // the start-of-step state is computed with the values the step used
+    const int rc = materialize_mid<T, QN>(h);
+    if (rc) return rc;
+  }
   LHIP_TRY(hipStreamSynchronize(h.stream));
   if (field == FVDBM_CELL_PDF) {
     // the current buffer only: the other keeps the start of the last step
@@ -1449,7 +1491,10 @@ int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms) {
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
   if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
-    int rc = ensure_graph(*h, h->cur);  // outside the timed region
+    // outside the timed region; both parities, since an odd run of
+    // two-step launches starts with one step
+    int rc = h->pairs ? ensure_graph2(*h, h->cur) : ensure_graph(*h, h->cur);
+    if (!rc && h->pairs) rc = ensure_graph2(*h, h->cur ^ 1);
     if (rc) return rc;
   }
   LHIP_TRY(hipEventRecord(h->ev0, h->stream));

@@ -1311,6 +1311,12 @@ struct Handle {
        *xpair_cell = nullptr, *xpair_w = nullptr, *pair_slot = nullptr;
   hipGraphExec_t gexec2[2] = {nullptr, nullptr};
   hipGraph_t graph2[2] = {nullptr, nullptr};
+  // two-step handles: after a run that ended in a two-step launch
+  // (prev_two_back), f[cur ^ 1] holds the pdf two steps back; the pdf at the
+  // start of the last step is one k_step from it, computed into fmid only
+  // when an observable is read (mid_ready)
+  void *fmid = nullptr;
+  bool prev_two_back = false, mid_ready = false;
   // all-node interpolation: node -> (slot, d) lists [N][K], cell -> vertex
   // CSR in slot order, and lazily allocated work arrays
   void *all_nc = nullptr, *all_nd = nullptr, *cn_off = nullptr, *cn_node = nullptr, *cn_d = nullptr;
@@ -1335,7 +1341,7 @@ struct Handle {
   double bgraph_tol = 0.0, bgraph_growth = 0.0;
   int cur = 0;           // buffer holding the current pdf
   int64_t steps = 0;     // steps taken since the last init/set of the cell pdf
-  bool has_prev = false; // f[cur^1] holds the pdf at the start of the last step
+  bool has_prev = false; // the pdf at the start of the last step is known (f[cur^1], or see fmid)
   bool observed = false; // observable buffers are up to date
   // graphs: one per starting buffer parity
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -1451,10 +1457,12 @@ static int grid_for(const Handle &h, int remap) {
   return remap ? ((h.ntiles + 7) / 8) * 8 : h.ntiles;
 }
 
-// Launch one step reading f[src]; appends to the stream.
+// Launch one step reading f[src] and writing f[src ^ 1] (or dst); appends
+// to the stream.
 template <typename T>
-static int launch_step(Handle &h, int src, bool resid) {
+static int launch_step(Handle &h, int src, bool resid, void *dst = nullptr) {
   StepArgs<T> a = make_args<T>(h, src);
+  if (dst) a.fdst = static_cast<T *>(dst);
   if (!a.inline_nodes && h.nan > 0) {
     // two-kernel variant: refresh active node pdfs first
     const int nb = (h.nan + 255) / 256;
@@ -1694,9 +1702,43 @@ static int run_batch_steps(Handle &h, int64_t n, bool resid_last) {
 static int run_steps(Handle &h, int64_t n, bool resid_last) {
   if (n <= 0) return FVDBM_OK;
   if (h.batch) return run_batch_steps(h, n, resid_last);
+  if (h.pairs && !resid_last && n >= 2) {
+    // two-step launches only; an odd count starts with one single step.
+    // f[cur ^ 1] then holds the pdf two steps back (see prev_two_back)
+    if (n & 1) {
+      int rc = launch_step_any(h, h.cur, false);
+      if (rc) return rc;
+      h.cur ^= 1;
+      ++h.steps;
+      --n;
+    }
+    int64_t rest = n - 2;
+    if (h.graph_steps >= 2) {
+      while (rest >= h.graph_steps) {
+        int rc = ensure_graph2(h, h.cur);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
+        h.cur ^= (h.graph_steps / 2) & 1;
+        rest -= h.graph_steps;
+        h.steps += h.graph_steps;
+      }
+    }
+    for (; rest >= 0; rest -= 2) {
+      int rc = launch_pair(h, h.cur);
+      if (rc) return rc;
+      h.cur ^= 1;
+      h.steps += 2;
+    }
+    h.prev_two_back = true;
+    h.mid_ready = false;
+    h.has_prev = true;
+    h.observed = false;
+    return FVDBM_OK;
+  }
   if (h.pairs && n >= 3) {
-    // pairs of steps for all but the last one or two, so the last step is
-    // a single launch and f[cur ^ 1] keeps the pdf at its start
+    // residual runs: pairs of steps for all but the last one or two, so the
+    // last (residual) step is a single launch and f[cur ^ 1] keeps the pdf
+    // at its start
     int64_t rest = n - 1;
     if (h.graph_steps >= 2) {
       while (rest >= h.graph_steps) {
@@ -1738,6 +1780,7 @@ static int run_steps(Handle &h, int64_t n, bool resid_last) {
     h.cur ^= 1;
     ++h.steps;
   }
+  h.prev_two_back = false;
   h.has_prev = true;
   h.observed = false;
   return FVDBM_OK;
@@ -2042,6 +2085,7 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
       if (!rc) rc = h.upload(&h.xpair_cell, L.xpair_cell);
       if (!rc) rc = h.upload(&h.xpair_w, L.xpair_w);
       if (!rc) rc = h.upload(&h.pair_slot, L.pair_slot);
+      if (!rc) rc = h.alloc(&h.fmid, (size_t)Q * h.Cp * es);
       if (!rc) {
         h.pairs = true;
         h.lds2 = lds2;
@@ -2094,7 +2138,21 @@ static int init_uniform_typed(Handle &h) {
   h.cur = 0;
   h.steps = 0;
   h.has_prev = false;
+  h.prev_two_back = false;
   h.observed = false;
+  return FVDBM_OK;
+}
+
+// After a run that ended in a two-step launch: the pdf at the start of the
+// last step, one step from f[cur ^ 1] (bitwise the two-step launch's
+// intermediate state), into fmid. Done before anything the step reads
+// (boundary-node values) changes, and at most once per run.
+template <typename T>
+static int materialize_mid(Handle &h) {
+  if (!h.has_prev || !h.prev_two_back || h.mid_ready) return FVDBM_OK;
+  const int rc = launch_step<T>(h, h.cur ^ 1, false, h.fmid);
+  if (rc) return rc;
+  h.mid_ready = true;
   return FVDBM_OK;
 }
 
@@ -2107,7 +2165,10 @@ static int observe_typed(Handle &h) {
   if (!rc && !h.obs_feq) rc = h.alloc(&h.obs_feq, (size_t)Q * h.Cp * sizeof(T));
   if (!rc && !h.obs_flux) rc = h.alloc(&h.obs_flux, (size_t)Q * h.F * sizeof(T));
   if (rc) return rc;
+  rc = materialize_mid<T>(h);
+  if (rc) return rc;
   StepArgs<T> a = make_args<T>(h, h.cur ^ 1);  // the pdf at the start of the last step
+  if (h.prev_two_back) a.fsrc = static_cast<const T *>(h.fmid);
   if (h.nan > 0)
     hipLaunchKernelGGL((k_nodes<T>), dim3((h.nan + 255) / 256), dim3(256), 0, h.stream, a,
                        static_cast<const int32_t *>(h.an_gid), static_cast<T *>(h.node_pdf),
@@ -2238,6 +2299,11 @@ static int set_typed(Handle &h, int field, const void *src) {
     return fail(FVDBM_ERR_INVALID, "a batched handle exposes FVDBM_CELL_PDF only");
   const T *in = static_cast<const T *>(src);
   const size_t es = sizeof(T);
+  if (field == FVDBM_NODE_PDF || field == FVDBM_NODE_VEL || field == FVDBM_NODE_RHO) {
+    // the start-of-step state is computed with the values the step used
+    const int rc = materialize_mid<T>(h);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamSynchronize(h.stream));
   if (field == FVDBM_CELL_PDF) {
     std::vector<T> tmp((size_t)Q * h.Cp, T(0));
@@ -2259,7 +2325,7 @@ static int set_typed(Handle &h, int field, const void *src) {
       return FVDBM_OK;
     }
     // write the current buffer only: f[cur^1] keeps the pdf at the start of
-    // the last step, which the derived cell fields still describe
+    // the last step (or two steps back), which the derived fields still describe
     HIP_TRY(hipMemcpy(h.f[h.cur], tmp.data(), tmp.size() * es, hipMemcpyHostToDevice));
     return FVDBM_OK;
   }
@@ -2550,13 +2616,26 @@ int fvdbm_batch_run(void *handle, int64_t max_steps, int32_t check_every, double
   if (!h || !n_active || !steps_run) return fail(FVDBM_ERR_INVALID, "null argument");
   if (!h->batch) return fail(FVDBM_ERR_STATE, "not a batched handle");
   if (check_every < 1 || max_steps < 0) return fail(FVDBM_ERR_INVALID, "bad check interval");
-  int rc = fvdbm::ensure_batch_graph(*h, check_every, tol, growth);
+  int rc = FVDBM_OK;
+  if (max_steps >= check_every) rc = fvdbm::ensure_batch_graph(*h, check_every, tol, growth);
   if (rc) return rc;
   int32_t na = h->ncases;
   int64_t done = 0;
-  while (done < max_steps && na > 0) {
+  while (done + check_every <= max_steps && na > 0) {
     HIP_TRY(hipGraphLaunch(h->bexec, h->stream));
     done += check_every;
+    HIP_TRY(hipMemcpyAsync(&na, h->n_active, sizeof(na), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  if (done < max_steps && na > 0) {
+    // a last, shorter interval (direct launches) so that no case runs past
+    // max_steps; its residual check is the same as a full interval's
+    const int rest = (int)(max_steps - done);
+    int swaps = 0;
+    rc = fvdbm::launch_batch_steps(*h, rest, true, &swaps);
+    if (!rc) rc = fvdbm::launch_batch_check(*h, tol, growth, rest, swaps);
+    if (rc) return rc;
+    done += rest;
     HIP_TRY(hipMemcpyAsync(&na, h->n_active, sizeof(na), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
   }
@@ -2718,9 +2797,11 @@ int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms) {
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   // build graphs outside the timed region
   if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
+    // both parities: an odd run of two-step launches starts with one step
     int rc = h->batch  ? fvdbm::ensure_batch_step_graph(*h)
              : h->pairs ? fvdbm::ensure_graph2(*h, h->cur)
                         : fvdbm::ensure_graph(*h, h->cur);
+    if (!rc && h->pairs && !h->batch) rc = fvdbm::ensure_graph2(*h, h->cur ^ 1);
     if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));

@@ -49,7 +49,9 @@ def solve(solver, init=None, tol=1e-6, check_every=500, max_steps=400_000, growt
     concatenated nodes. With ``gate``, a case whose predicted field is not
     ``plausible`` starts cold at once. A warm case that diverges or grows
     (``growth`` times its smallest residual) is restarted cold from W; its
-    step count keeps the warm steps it spent. Returns dict: steps, status,
+    step count keeps the warm steps it spent, and the cold restarts share
+    what is left of ``max_steps`` after the longest of those warm attempts,
+    so no case runs more than ``max_steps`` in all. Returns dict: steps, status,
     residual (per case), fallback (bool per case: started or restarted
     cold), wall_s (seconds, including the initialisation)."""
     t0 = time.perf_counter()
@@ -74,7 +76,9 @@ def solve(solver, init=None, tol=1e-6, check_every=500, max_steps=400_000, growt
             fallback |= bad
             _set_uniform(solver, np.nonzero(bad)[0])
             solver.set_active(bad.astype(np.int32), reset_steps=False)
-            st = solver.run(max_steps, check_every, tol, 0.0)
+            left = int(max_steps - st["steps"][bad].max())
+            if left > 0:
+                st = solver.run(left, check_every, tol, 0.0)
     if synchronize is not None:
         synchronize()
     return dict(steps=st["steps"].copy(), status=st["status"].copy(), residual=st["residual"].copy(),

@@ -198,7 +198,9 @@ int fvdbm_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy);
 int fvdbm_create_batch(const fvdbm_mesh_desc *mesh, const fvdbm_params *params,
                        const fvdbm_batch_desc *batch, void **handle);
 /* Run active cases in intervals of check_every steps (one captured graph
- * per interval) until none is active or max_steps have run. After each
+ * per interval) until none is active or max_steps have run; when
+ * max_steps is not a multiple of check_every the last interval is the
+ * shorter remainder, so no case runs more than max_steps. After each
  * interval's last step a case stops as CONVERGED when its residual
  * ||u^{n+1}-u^n||/||u^{n+1}|| is below tol, DIVERGED when it is not finite,
  * GROWING when growth > 0 and it exceeds growth times its smallest residual

@@ -72,14 +72,15 @@ class OracleBatch:
     def run(self, max_steps, check_every=100, tol=1e-6, growth=0.0):
         done = 0
         while done < max_steps and self.active.any():
+            every = min(check_every, max_steps - done)  # the last interval may be shorter
             for k in np.nonzero(self.active)[0]:
                 c, e, s = self.cases[k], self.envs[k], self.states[k]
                 tau, dt = c.dynamics.tau, c.dynamics.delta_t
-                oracle.run(e, s, tau, dt, check_every - 1)
+                oracle.run(e, s, tau, dt, every - 1)
                 u0 = _vel(s["cell_pdf"])
                 oracle.run(e, s, tau, dt, 1)
                 u1 = _vel(s["cell_pdf"])
-                self.steps[k] += check_every
+                self.steps[k] += every
                 with np.errstate(invalid="ignore"):
                     r = np.linalg.norm(u1 - u0) / np.linalg.norm(u1)
                 self.resid[k] = r
@@ -90,7 +91,7 @@ class OracleBatch:
                 elif growth > 0 and r > growth * self.best[k]:
                     self.st[k], self.active[k] = GROWING, False
                 self.best[k] = min(self.best[k], r)
-            done += check_every
+            done += every
         return self.status()
 
     def status(self):

@@ -69,6 +69,41 @@ def test_lds_classes_equal_one_launch(gpu):
     assert 2 in classes, classes
 
 
+def test_workload_batch_matches_oracle(gpu):
+    """Three cases of the batched throughput workload (5-40 discs, 20k-47k
+    cells, velocity inlet, density outlet, each its own tau and dt) in one
+    batch, at three tile sizes: after 100 steps every case's pdf is bitwise
+    equal to the oracle's single solve of it. At one tile size at least,
+    these boundary-heavy meshes take the two-launch split by tile LDS size,
+    so both launch classes are checked against the oracle."""
+    from cases import oracle_env, oracle_state
+    from oracle import oracle
+    wl = fv.porous.WORKLOAD
+    cases = [fv.porous.make_case(seed, dist=wl, target_cells=t)
+             for seed, t in ((11, 20_000), (12, 31_000), (13, 47_000))]
+    for c in cases:
+        assert wl["count_range"][0] <= len(c.discs) <= wl["count_range"][1]
+        assert c.n_cells >= 19_000
+    expect = []
+    for c in cases:
+        oenv = oracle_env(c.cells, c.faces, c.nodes)
+        st = oracle_state(oenv, c.nodes)
+        oracle.run(oenv, st, c.dynamics.tau, c.dynamics.delta_t, 100, "f32")
+        assert np.isfinite(st["cell_pdf"]).all()
+        expect.append(st["cell_pdf"])
+    classes = []
+    for block in (64, 128, 256):
+        print(f"workload batch, block {block}", flush=True)
+        b = batch.BatchSolver([c.env_tuple() for c in cases], block=block)
+        classes.append(b.info()["launch_classes"])
+        b.step(100)
+        pdf = b.get_pdf()
+        for k, want in enumerate(expect):
+            assert np.array_equal(b.case_pdf(k, pdf), want), (block, k)
+        b.close()
+    assert 2 in classes, classes
+
+
 @pytest.mark.parametrize("variant", [0, 2, 8])
 def test_run_residuals_and_freezing(gpu, variant):
     """variant 0 runs two steps per launch (two-step tiles), 8 one step per
@@ -98,6 +133,22 @@ def test_run_residuals_and_freezing(gpu, variant):
     assert np.array_equal(b.case_pdf(2), single)
     assert np.array_equal(b.case_pdf(1), frozen)
     assert b.status()["steps"].tolist() == [308, 200, 308]
+
+
+def test_run_stops_at_max_steps(gpu):
+    """A step cap that is not a multiple of the check interval ends with a
+    shorter interval: every case runs exactly max_steps, its pdf and last
+    residual are those of a single solve of that many steps."""
+    cs = _cases(2)
+    b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs])
+    st = b.run(max_steps=130, check_every=50, tol=0.0)
+    assert b.steps_run == 130 and st["steps"].tolist() == [130, 130]
+    for k, (m, dyn, c, f, n) in enumerate(cs):
+        single, r = _single(m, dyn, c, f, n, 130, resid=True)
+        assert np.array_equal(b.case_pdf(k), single)
+        assert st["residual"][k] == pytest.approx(r, rel=1e-12)
+    st = b.run(max_steps=20, check_every=50, tol=0.0)  # a cap below one interval
+    assert b.steps_run == 20 and st["steps"].tolist() == [150, 150]
 
 
 def test_run_until_converged(gpu):

@@ -64,6 +64,21 @@ def test_d2q13_cavity_matches_oracle(gpu, precision):
     assert np.isfinite(env.cells.pdf).all()
 
 
+def test_d2q13_observables_after_runs_of_every_length(gpu):
+    """Runs of two or more steps end in a two-step launch that keeps its
+    intermediate state for the start-of-step observables; every observable
+    equals the oracle's after runs of 2, 3, 1, 4 and 5 steps."""
+    m, dyn, cells, faces, nodes = _channel13(target_cells=6_000)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes, "f32", 13)
+    env = fv.Environment(cells, faces, nodes)
+    env.init()
+    for n in (2, 3, 1, 4, 5):
+        oracle.run(oenv, st, dyn.tau, dyn.delta_t, n, "f32", quivers=13)
+        env.step(n)
+        _equal(env, st, f"run of {n}")
+
+
 @pytest.mark.parametrize("scheme", ["upwind", "lax_wendroff"])
 def test_d2q13_channel_matches_oracle(gpu, scheme):
     m, dyn, cells, faces, nodes = _channel13(scheme=scheme)

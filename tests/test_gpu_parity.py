@@ -117,6 +117,27 @@ def test_channel_matches_oracle(gpu, scheme):
     _assert_fields_equal(env, st)
 
 
+def test_full_size_channel_matches_oracle(gpu):
+    """The benchmarked workload itself: the 1M-cell cylinder channel (seed 0,
+    tau 0.55, u_max 0.1, dim_multiplier 1/mean(A), dt at the explicit bound;
+    the geometry, tile count, XCD-aware tile order and block-256 LDS sizing
+    of the bench), 50 steps, every observable bitwise equal to the OpenMP
+    oracle."""
+    import os
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    m, dyn, cells, faces, nodes = _channel(target_cells=1_000_000, seed=0)
+    assert 990_000 <= len(m.cells) <= 1_010_000
+    print(f"full-size channel: {len(m.cells)} cells", flush=True)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 50, "f32")
+    env = _gpu_env(cells, faces, nodes)
+    info = env.info()
+    assert not (info["variant"] & 8) and info["block"] == 256  # two-step tiles, bench tile size
+    env.step(50)
+    _assert_fields_equal(env, st, what="1M channel, 50 steps")
+
+
 @pytest.mark.parametrize("opts", [
     dict(reorder=0), dict(reorder=1), dict(reorder=2), dict(reorder=3), dict(variant=1), dict(variant=2),
     dict(variant=3), dict(variant=4), dict(variant=7), dict(reorder=3, block=1024),
@@ -174,6 +195,45 @@ def test_moments_describe_start_of_last_step(gpu):
     assert np.array_equal(env.cells.rho.reshape(-1), rho)
     assert np.array_equal(env.cells.vel, vel)
     assert np.array_equal(env.cells.pdf_eq, oracle.calc_eq(rho, vel))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_observables_after_runs_of_every_length(gpu, precision):
+    """Runs of two or more steps end in a two-step launch that keeps its
+    intermediate state for the start-of-step observables; odd runs start
+    with one single step. Every observable equals the oracle's after runs
+    of 1, 2, 3, 4, 7, 2 and 66 steps (66 crosses a captured graph)."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=4)
+    p = "f32" if precision == "fp32" else "f64"
+    oenv = oracle_env(cells, faces, nodes, p)
+    st = oracle_state(oenv, nodes, p)
+    env = _gpu_env(cells, faces, nodes, precision)
+    for n in (1, 2, 3, 4, 7, 2, 66):
+        oracle.run(oenv, st, dyn.tau, dyn.delta_t, n, p)
+        env = env.step(n)
+        _assert_fields_equal(env, st, what=f"run of {n}")
+
+
+def test_boundary_values_set_after_a_two_step_run(gpu):
+    """Velocity-node values changed after a run that ended in a two-step
+    launch: the cell observables still describe the start of the last step
+    as that run computed it (with the old values), and the following steps
+    use the new values."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=5)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    env = _gpu_env(cells, faces, nodes)
+    env.step(4)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 4, "f32")
+    vel = np.asarray(st["node_vel"], np.float32).copy()
+    vnode = nodes.type.reshape(-1) == 1
+    vel[vnode] *= np.float32(1.5)
+    env.nodes.vel = vel
+    _assert_fields_equal(env, st, FIELDS[:4], "after setting node velocities")
+    st["node_vel"][vnode] = vel[vnode]
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 3, "f32")
+    env.step(3)
+    _assert_fields_equal(env, st, what="3 steps with the new values")
 
 
 def test_every_cell_written_and_deterministic(gpu):

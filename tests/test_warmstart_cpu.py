@@ -70,3 +70,14 @@ def test_forced_residual_growth_falls_back(cold):
     assert out["status"].tolist() == [CONVERGED]
     assert out["steps"][0] == res["steps"][0] + 2 * EVERY
     assert np.array_equal(b.get_pdf(), pdf)
+
+
+def test_fallback_keeps_within_the_step_budget(cold):
+    """A warm case that grows after two intervals restarts cold with what is
+    left of max_steps: three intervals in all, never more."""
+    case, res, pdf, (nr, nv) = cold
+    b = OracleBatch([case])
+    out = ws.solve(b, (nr, nv), tol=TOL, check_every=EVERY, max_steps=3 * EVERY + 17,
+                   growth=1e-12)
+    assert out["fallback"].tolist() == [True]
+    assert out["steps"][0] == 3 * EVERY + 17
