@@ -107,6 +107,8 @@ _SIGNATURES = {
                                          ct.c_void_p, ct.c_void_p]),
     "fvdbm_nn_wgrad": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_int32,
                                   ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "fvdbm_nn_wgrad_noalloc": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int32,
+                                          ct.c_int32, ct.c_void_p, ct.c_void_p]),
     "fvdbm_nn_colsum": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int32, ct.c_void_p, ct.c_void_p,
                                    ct.c_void_p]),
     "fvdbm_nn_aggregate": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p,

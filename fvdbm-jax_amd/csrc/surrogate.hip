@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "fvdbm.h"
@@ -395,6 +396,47 @@ static void launch_wgrad(const float *a, const float *b, int m, float *partial, 
   hipLaunchKernelGGL((k_wgrad<KA, NB>), dim3(nch), dim3(256), 0, s, a, b, m, partial);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free weight gradient (fvdbm_nn_wgrad_noalloc): the per-chunk
+// partial products live in this file-scope device array instead of a
+// caller workspace. Each GPU holds its own copy for the life of the
+// process. Its host-side name is not a device address: the address on the
+// current device comes from hipGetSymbolAddress, looked up once per device
+// and kept in scratch_dev. k_wgrad writes every word of the partials it
+// reduces, so nothing needs clearing between calls; calls that overlap on
+// one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchFloats = size_t(4) << 20;  // 16 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static float *scratch_dev[kMaxDevices] = {};
+
+// The scratch array's address on the current device (looked up on a
+// device's first call).
+static int scratch_on_current_device(float **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    g_err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return FVDBM_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return nn_fail("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return FVDBM_ERR_HIP;
+    }
+    scratch_dev[dev] = static_cast<float *>(p);
+  }
+  *out = scratch_dev[dev];
+  return FVDBM_OK;
+}
+
 }  // namespace fvdbm_nn
 
 using namespace fvdbm_nn;
@@ -462,6 +504,39 @@ int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_
   else return nn_fail("unsupported wgrad widths");
   const int n = ka * nb;
   hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, nch, n, out);
+  return hip_status("wgrad launch");
+}
+
+int fvdbm_nn_wgrad_noalloc(const float *a, const float *b, int64_t m, int32_t ka, int32_t nb,
+                           float *out, void *stream) {
+  if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
+  if (!a || !b || !out) return nn_fail("null pointer");
+  if (!((ka == 16 && nb == 64) || (ka == 64 && nb == 64) || (ka == 256 && nb == 64) ||
+        (ka == 64 && nb == 16)))
+    return nn_fail("unsupported wgrad widths");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mi = (int)m;
+  const int n = ka * nb;
+  const int64_t nch = (m + CHUNK - 1) / CHUNK;
+  if ((uint64_t)nch * (uint64_t)n > kScratchFloats)
+    return nn_fail("fvdbm_nn_wgrad_noalloc: " + std::to_string(m) + " rows need " +
+                   std::to_string(nch * n) + " partial floats, more than the " +
+                   std::to_string(kScratchFloats) + " of the static scratch (at most " +
+                   std::to_string((int64_t)(kScratchFloats / n) * CHUNK) + " rows for widths (" +
+                   std::to_string(ka) + ", " + std::to_string(nb) + "))");
+  if (mi == 0) {
+    hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, s, out, n);
+    return hip_status("zero launch");
+  }
+  float *partial = nullptr;
+  const int rc = scratch_on_current_device(&partial);
+  if (rc) return rc;
+  if (ka == 16) launch_wgrad<16, 64>(a, b, mi, partial, s);
+  else if (ka == 256) launch_wgrad<256, 64>(a, b, mi, partial, s);
+  else if (nb == 64) launch_wgrad<64, 64>(a, b, mi, partial, s);
+  else launch_wgrad<64, 16>(a, b, mi, partial, s);
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, (int)nch, n,
+                     out);
   return hip_status("wgrad launch");
 }
 

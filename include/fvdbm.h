@@ -272,6 +272,18 @@ int fvdbm_nn_mlp_backward(const float *x, int64_t m, int32_t k, const float *w1,
  * Deterministic: per-chunk partial products, summed in chunk order. */
 int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_t nb, float *partial,
                    float *out, void *stream);
+/* As fvdbm_nn_wgrad, with no workspace argument and no allocation. No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). The chunk partials go to a static 16 MiB device
+ * array (one per GPU, held until the process ends), so
+ * ceil(m / fvdbm_nn_chunk_rows()) * ka * nb must not exceed 4194304: at
+ * most 65536 rows for (256, 64), 262144 for (64, 64) and 1048576 for
+ * (16, 64) and (64, 16); larger m fails with FVDBM_ERR_INVALID. Calls that
+ * overlap on one GPU share the array and race. Results equal
+ * fvdbm_nn_wgrad's bit for bit (the same kernels). */
+int fvdbm_nn_wgrad_noalloc(const float *a, const float *b, int64_t m, int32_t ka, int32_t nb,
+                           float *out, void *stream);
 /* out [n] = column sums of a [m][n] (n <= 256), deterministic as above. */
 int fvdbm_nn_colsum(const float *a, int64_t m, int32_t n, float *partial, float *out, void *stream);
 /* Mesh aggregation over a CSR edge list (row_off [n+1], col [E]):
