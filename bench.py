@@ -8,32 +8,42 @@ reference), so dim_multiplier = 1/mean(cell area) and dt is the explicit
 stability bound at CFL 0.5; both are reported.
 
 Steps run as two-step launches (k_step2: each advances every cell by two
-steps, the intermediate state held in LDS) and a final single step
-(k_step). The timed region is K steps between two HIP events on the
-solver's stream (launched as captured hipGraphs), with the inputs resident
-in HBM. value = cells * K / elapsed / 1e6 (MLUPS). roofline.achieved = the
+steps, the intermediate state held in LDS; an odd count starts with one
+single step, k_step). The timed region is K steps between two HIP events
+on the solver's stream (launched as captured hipGraphs where K allows),
+with the inputs resident in HBM. value = cells * K / elapsed / 1e6 (MLUPS). roofline.achieved = the
 algorithmic 108 B per cell update * cells * 2 per k_step2 launch / the
 launch's duration, i.e. 108 B/LUP * cells / (elapsed/K).
 cpu_baseline = the C oracle (OpenMP over the element loops, min(16, CPUs)
 threads) on a bounded number of steps of the same mesh, on this host.
 
+fp32_vs_fp64: relative L2 of rho - 1 and u against the same kernels in FP64
+after a fixed number of steps from the uniform state: the channel after
+``--fp64-steps`` (10k) steps, the 32x32 cavity after 1000; the batched and
+steady-state lines carry their own.
+
 batched: ``--batch-cases`` (default 256) porous cases of the batched
 workload (5-40 discs, 20k-200k cells, seeds 0..) in one handle: the
 autotuner picks tile size and kernel variant on the first 32 cases (every
 candidate checked bitwise), then MLUPS of the batched step over all their
-cells, then a masked run to a 1e-6 residual with a step cap (many of these
-cases have no steady state).
+cells, then a masked run to a 1e-6 residual capped at ``--masked-steps``
+(200k) steps, with a runaway stop (mean density past 2: most of these
+channels are choked and have no steady state), reported as converged /
+runaway / still-running counts; FP32 against FP64 on 16 cases after 20k
+steps.
 
-steady_state (the second half of the metric): ``--steady-cases`` unseen
-porous-channel cases of the study distribution (seeds from 1,000,000; the
-surrogate was trained on seeds below 10,000) solved as one batch to a
-residual of 1e-6, first from the uniform state (cold), then from the
-surrogate's prediction (warm, cold fallback per case). Reported: steps per
-case, batch wall time (the warm time includes meshing-independent feature
-preparation and inference), and the relative L2 distance of the final
-cell fields.
+steady_state (the second half of the metric): ``--steady-cases`` (64)
+unseen porous channels of the steady-state distribution (3-8 discs,
+20k-200k cells; seeds from 1,000,000, the surrogate's training seeds lie
+below) solved as one batch to a per-step residual of 1e-6, first from the
+uniform state (cold), then from the surrogate's prediction (warm, cold
+fallback per case). Reported: steps per case against the 3x target, batch
+wall time (the warm time includes input preparation and inference), and the
+final fields: warm against cold, and both against an FP64 reference (the
+first ``--steady-ref-cases`` cases, the cold solution continued in FP64 to
+a residual of 1e-10) against the 1e-4 target.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--steady-cases M]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--parts single,batched,steady]
 """
 from __future__ import annotations
 
@@ -148,12 +158,52 @@ def _build_workload(seed):
     return fv.porous.make_case(seed, dist=fv.porous.WORKLOAD).env_tuple()
 
 
-def batched(triples, steps=1000, masked_steps=20_000, check_every=500, tune_cases=32):
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _moment_errors(r32, u32, r64, u64):
+    return dict(rel_l2_u=_rel(u32, u64), rel_l2_rho=_rel(np.asarray(r32, np.float64) - 1.0,
+                                                        np.asarray(r64, np.float64) - 1.0))
+
+
+def fp64_single(fv, c, f, n, steps, **opts):
+    """rho and u after ``steps`` steps from the uniform state, FP32 against
+    the same kernels in FP64."""
+    out = {}
+    for prec in ("fp32", "fp64"):
+        e = fv.Environment(c, f, n, precision=prec, **opts)
+        e.init()
+        e.step(steps)
+        out[prec] = _moments(e)
+        e.close()
+    return dict(steps=steps, **_moment_errors(*out["fp32"], *out["fp64"]))
+
+
+def fp64_cavity(fv):
+    """The 32x32 cavity (tau 0.6, dt 0.01, lid u 0.1), 1000 steps."""
+    mg = fv.meshgen
+    m = fv.Mesher.from_raw(mg.ldc_structured(32))
+    m.enforce_ccw()
+    m.calc_mesh_properties()
+    dyn = fv.D2Q9(tau=0.6, delta_t=0.01)
+    c, f, n = m.to_env(dyn, "upwind", dim_multiplier=1)
+    n = m.set_vel_node(n, mg.LID, np.array([0.1, 0.0]))
+    n = m.set_vel_node(n, mg.WALL, np.array([0.0, 0.0]))
+    return fp64_single(fv, c, f, n, 1000)
+
+
+def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_cases=32,
+            fp64_cases=16, fp64_steps=20_000):
     """Throughput of one batched launch over uneven porous cases (the
     batched workload's distribution), then a masked run: every case steps
-    until its residual is below 1e-6, it diverges or grows 10x past its best,
-    or ``masked_steps`` have run. The launch options are tuned on the first
-    ``tune_cases`` cases (every candidate checked bitwise)."""
+    until its residual is below 1e-6 (converged), its mean density passes 2
+    (runaway: a choked channel has no steady state) or ``masked_steps`` have
+    run. The launch options are tuned on the first ``tune_cases`` cases
+    (every candidate checked bitwise). FP32 against FP64 on the first
+    ``fp64_cases`` cases after ``fp64_steps`` steps (per-case medians)."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     _lib = importlib.import_module("fvdbm-jax_amd._lib")
@@ -169,8 +219,7 @@ def batched(triples, steps=1000, masked_steps=20_000, check_every=500, tune_case
     b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
     log(f"batched handle: {len(triples)} cases uploaded in {time.perf_counter() - t0:.1f}s")
     cells = int(sum(c.face_indices.shape[0] for c, _, _ in triples))
-    info = _lib.Info()
-    _lib.check(_lib.lib().fvdbm_info_get(b._handle(), ct.byref(info)), "fvdbm_info_get")
+    info = b.info()
     b.step(100)
     ms = ct.c_double()
     _lib.check(_lib.lib().fvdbm_step_timed(b._handle(), int(steps), ct.byref(ms)), "fvdbm_step_timed")
@@ -178,26 +227,49 @@ def batched(triples, steps=1000, masked_steps=20_000, check_every=500, tune_case
     mlups = cells / per / 1e6
     b.reset_uniform()
     b.set_active(np.ones(b.n, np.int32), reset_steps=True)
-    t0 = time.perf_counter()
-    st = b.run(masked_steps, check_every, 1e-6, 10.0)
-    wall = time.perf_counter() - t0
+    st = ws.run_until(b, masked_steps, check_every, 1e-6, rho_max=ws.RUNAWAY_RHO, chunk=10_000,
+                      log=lambda m: log("masked: " + m))
     b.close()
-    counts = {ws_name: int((st["status"] == code).sum()) for code, ws_name in
-              ((ws.RUNNING, "running"), (ws.CONVERGED, "converged"), (ws.DIVERGED, "diverged"),
-               (ws.GROWING, "growing"))}
-    return dict(cases=len(triples), cells=cells, padded_cells=int(info.n_cells_padded), steps=steps,
+    conv = st["status"] == ws.CONVERGED
+    running = st["active"]
+    masked = dict(max_steps=masked_steps, check_every=check_every, runaway_mean_rho=ws.RUNAWAY_RHO,
+                  wall_s=round(st["wall_s"], 2), converged=int(conv.sum()),
+                  runaway=int(st["runaway"].sum()), running=int(running.sum()),
+                  diverged=int((st["status"] == ws.DIVERGED).sum()),
+                  steps_converged_median=float(np.median(st["steps"][conv])) if conv.any() else None,
+                  steps_runaway_median=float(np.median(st["steps"][st["runaway"]]))
+                  if st["runaway"].any() else None)
+    # FP32 against FP64 at a fixed horizon, per case
+    sub = triples[:fp64_cases]
+    res = {}
+    for prec in ("fp32", "fp64"):
+        bb = batch.BatchSolver(sub, precision=prec, block=best["block"], variant=best["variant"],
+                               reorder=best["reorder"])
+        bb.step(fp64_steps)
+        r, u = bb.cell_fields()
+        res[prec] = [(bb.case_cells(k, r), bb.case_cells(k, u)) for k in range(bb.n)]
+        bb.close()
+    errs = [_moment_errors(*res["fp32"][k], *res["fp64"][k]) for k in range(len(sub))]
+    fp64 = dict(cases=len(sub), steps=fp64_steps,
+                rel_l2_u_median=float(np.median([e["rel_l2_u"] for e in errs])),
+                rel_l2_u_max=float(np.max([e["rel_l2_u"] for e in errs])),
+                rel_l2_rho_median=float(np.median([e["rel_l2_rho"] for e in errs])))
+    return dict(cases=len(triples), cells=cells, padded_cells=int(info["n_cells_padded"]), steps=steps,
+                launch_classes=int(info["launch_classes"]),
                 tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"],
                            on_cases=min(tune_cases, len(triples))),
                 default_us_per_step=default[0]["us_per_step"] if default else None,
                 us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
                 hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
-                masked=dict(max_steps=masked_steps, wall_s=round(wall, 2), **counts,
-                            steps_median=float(np.median(st["steps"]))))
+                masked=masked, fp32_vs_fp64=fp64)
 
 
-def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
-    """Cold against warm-started batched solves of the same cases."""
-    fv = importlib.import_module("fvdbm-jax_amd")
+def steady_state(cases, weights, tol=1e-6, check_every=1000, max_steps=800_000, ref_cases=16,
+                 ref_tol=1e-10):
+    """Cold against warm-started batched solves of the same unseen cases,
+    both in FP32 to a per-step residual below ``tol``; then an FP64 run of
+    the first ``ref_cases`` cases, continued from the cold solution to a
+    residual below ``ref_tol``, judges both final fields."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     sg = importlib.import_module("fvdbm-jax_amd.surrogate")
@@ -205,6 +277,7 @@ def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
     # a short run first so graph capture and first-touch are outside both timings
     ws.solve(b, None, tol=tol, check_every=check_every, max_steps=2 * check_every)
     cold = ws.solve(b, None, tol=tol, check_every=check_every, max_steps=max_steps)
+    cold_pdf = b.get_pdf()
     cr, cv = b.cell_fields()
     # the best a node-field predictor can do: restart from the converged
     # solution's own node fields
@@ -223,8 +296,33 @@ def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
     ok = (cold["status"] == ws.CONVERGED) & (warm["status"] == ws.CONVERGED)
     ratio = cold["steps"][ok] / np.maximum(warm["steps"][ok], 1)
     warm_s = warm["wall_s"] + t_pred
+    # FP64 reference: the cold FP32 solution continued in FP64 to ref_tol
+    nref = min(ref_cases, len(cases))
+    ref = None
+    if nref > 0:
+        t1 = time.perf_counter()
+        rb = batch.BatchSolver([c.env_tuple() for c in cases[:nref]], precision="fp64")
+        rb.set_pdf(cold_pdf[:int(rb.cell_off[-1])].astype(np.float64))
+        rb.set_active(np.ones(rb.n, np.int32), reset_steps=True)
+        rs = rb.run(4_000_000, check_every, ref_tol, 0.0)
+        rr, rv = rb.cell_fields()
+        e = {}
+        for name, (fr, fvv) in (("cold", (cr, cv)), ("warm", (wr, wv))):
+            e[name] = np.array([ws.relative_l2(b.case_cells(k, fvv), rb.case_cells(k, rv))
+                                for k in range(nref)])
+            e[name + "_rho"] = np.array([ws.relative_l2(b.case_cells(k, fr) - 1.0,
+                                                        rb.case_cells(k, rr) - 1.0) for k in range(nref)])
+        rb.close()
+        ref = dict(cases=nref, tol=ref_tol, converged=int((rs["status"] == ws.CONVERGED).sum()),
+                   extra_steps_median=float(np.median(rs["steps"])), wall_s=round(time.perf_counter() - t1, 2),
+                   rel_l2_u_cold_median=float(np.median(e["cold"])), rel_l2_u_cold_max=float(e["cold"].max()),
+                   rel_l2_u_warm_median=float(np.median(e["warm"])), rel_l2_u_warm_max=float(e["warm"].max()),
+                   rel_l2_rho_cold_median=float(np.median(e["cold_rho"])),
+                   rel_l2_rho_warm_median=float(np.median(e["warm_rho"])))
+    cells = np.array([c.n_cells for c in cases])
     return dict(
-        cases=len(cases), cells=int(sum(c.n_cells for c in cases)), tol=tol, check_every=check_every,
+        cases=len(cases), cells=int(cells.sum()), cells_per_case=[int(cells.min()), int(cells.max())],
+        tol=tol, check_every=check_every,
         converged_cold=int((cold["status"] == ws.CONVERGED).sum()),
         converged_warm=int((warm["status"] == ws.CONVERGED).sum()),
         fallbacks=int(warm["fallback"].sum()),
@@ -233,10 +331,13 @@ def steady_state(cases, weights, tol=1e-6, check_every=500, max_steps=400_000):
         steps_exact_fields_median=float(np.median(ideal["steps"])),
         step_ratio_median=float(np.median(ratio)) if ratio.size else None,
         step_ratio_total=float(cold["steps"].sum() / max(warm["steps"].sum(), 1)),
+        step_ratio_target=3.0,
         wall_cold_s=round(cold["wall_s"], 3), wall_warm_s=round(warm_s, 3),
         inference_s=round(t_pred, 3), wall_ratio=round(cold["wall_s"] / warm_s, 3),
-        final_rel_l2_u_median=float(np.median(du)), final_rel_l2_u_max=float(du.max()),
-        final_rel_l2_rho_median=float(np.median(dr)), final_rel_l2_rho_max=float(dr.max()))
+        final_rel_l2_u_warm_vs_cold_median=float(np.median(du)), final_rel_l2_u_warm_vs_cold_max=float(du.max()),
+        final_rel_l2_rho_warm_vs_cold_median=float(np.median(dr)),
+        final_rel_l2_target=1e-4,
+        fp64_reference=ref)
 
 
 def load_traffic(workload):
@@ -260,34 +361,48 @@ def main():
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--reorder", type=int, default=None)
+    ap.add_argument("--parts", default="single,batched,steady",
+                    help="comma list of: single (the 1M channel line), batched, steady")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=10_000,
+                    help="steps of the FP32-against-FP64 comparison on the channel")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 256 MiB device copy (counter calibration)")
     ap.add_argument("--mesh-cache", default=None,
                     help="directory to keep the generated mesh between runs")
-    ap.add_argument("--steady-cases", type=int, default=1024,
-                    help="unseen porous cases for the cold/warm steady-state study (0: skip)")
+    ap.add_argument("--steady-cases", type=int, default=64,
+                    help="unseen porous cases (20k-200k cells) for the cold/warm steady-state study (0: skip)")
+    ap.add_argument("--steady-ref-cases", type=int, default=16,
+                    help="of those, cases that get the FP64 reference solution")
     ap.add_argument("--steady-seed0", type=int, default=1_000_000)
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="porous cases of the batched workload for the batched throughput line (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000, help="timed steps of the batched launch")
+    ap.add_argument("--masked-steps", type=int, default=200_000,
+                    help="step cap of the batched masked run (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-
+    parts = set(p.strip() for p in args.parts.split(",") if p.strip())
+    if parts - {"single", "batched", "steady"}:
+        raise SystemExit(f"unknown --parts {sorted(parts - {'single', 'batched', 'steady'})}")
     if args.lattice != "d2q9":  # the batched and steady-state parts run on D2Q9 handles
-        args.steady_cases = args.batch_cases = 0
+        parts &= {"single"}
+
     steady_cases = None
-    if args.steady_cases > 0 and os.path.exists(args.weights):
+    if "steady" in parts and args.steady_cases > 0 and os.path.exists(args.weights):
         # meshed in a fork pool before anything touches the GPU
+        fvp = importlib.import_module("fvdbm-jax_amd")
         ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
         t0 = time.time()
-        steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases))
+        steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases),
+                                      dist=fvp.porous.STEADY)
         log(f"steady-state study: {len(steady_cases)} cases meshed in {time.time() - t0:.1f}s")
     batch_cases = None
-    if args.batch_cases > 0:
+    if "batched" in parts and args.batch_cases > 0:
         import multiprocessing as mp
         t0 = time.time()
         with mp.get_context("fork").Pool(16) as pool:
@@ -305,95 +420,93 @@ def main():
             t = _lib.copy_probe(mib << 20, 50)
             copy[f"GBps_{mib}MiB"] = round(2 * (mib << 20) / (t / 1e3) / 1e9, 1)
         log(f"copy probe by size: {copy}")
-    fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache,
-                                           lattice=args.lattice)
-    C = len(m.cells)
-    env = fv.Environment(c, f, n, precision=args.precision, variant=args.variant, block=args.block,
-                         reorder=args.reorder)
-    t0 = time.time()
-    env.init()
-    info = env.info()
-    log(f"upload {time.time() - t0:.1f}s; bandwidth {info['bandwidth']}, boundary cells "
-        f"{info['n_boundary_cells']}, device bytes {info['device_bytes'] / 1e6:.1f} MB")
-    env.step(args.warmup)
-    env.synchronize()
-    ms = env.time_steps(args.steps)
-    if getattr(env, "_lattice", False):  # the any-lattice handle has no residual kernel
-        env.step(1)
-        r = None
-        if not np.isfinite(env.cells.pdf).all():
-            raise SystemExit("non-finite pdf: the run diverged")
-    else:
-        r = env.step_residual(1)
-        if not np.isfinite(r):
-            raise SystemExit("non-finite residual: the run diverged")
-    mlups = C * args.steps / (ms / 1e3) / 1e6
-    # two-step tiles unless the handle runs one step per launch (variant bit 8)
-    pairs = not (int(info["variant"]) & 8)
-    lattice_handle = getattr(env, "_lattice", False)
-    kernel = ("k_lat_step2" if pairs else "k_lat_step") if lattice_handle else ("k_step2" if pairs else "k_step")
-    updates_per_launch = 2 if pairs else 1
-    per_launch_s = ms / 1e3 / args.steps * updates_per_launch
-    bpl = bytes_per_lup(int(dyn.NUM_QUIVERS), args.precision)
-    achieved = bpl * C * updates_per_launch / per_launch_s / 1e9
-    workload = f"cylinder_channel_{C // 1000}k" if C < 999_500 else f"cylinder_channel_{round(C / 1e6)}M"
-    tr = load_traffic("cylinder_channel_1M") if abs(C - 1_000_000) < 50_000 else None
-    if tr is not None and (tr.get("kernel") != kernel or args.precision != "fp32"):
-        tr = None  # the committed counters are for another kernel or precision
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4),
-                traffic=(tr["bytes_per_launch"] if tr else None))
-    cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
-    fp64 = None
-    if args.precision == "fp32":
-        # the FP32 error of this run: the same steps with the same kernels in FP64
-        total = args.warmup + args.steps + 1
-        r32, u32 = _moments(env)
-        e64 = fv.Environment(c, f, n, precision="fp64", variant=args.variant, block=args.block,
-                             reorder=args.reorder)
-        e64.init()
-        e64.step(total)
-        r64, u64 = _moments(e64)
-        e64.close()
-        du = np.linalg.norm(u32.astype(np.float64) - u64) / np.linalg.norm(u64)
-        dr = np.linalg.norm(r32.astype(np.float64) - r64) / np.linalg.norm(r64)
-        fp64 = dict(steps=total, rel_l2_u=float(du), rel_l2_rho=float(dr))
-        log(f"fp32 vs fp64 after {total} steps: u {du:.3e}, rho {dr:.3e} (relative L2)")
-    env.close()
-    bat = None
-    if batch_cases is not None:
-        t0 = time.time()
-        bat = batched(batch_cases)
-        log(f"batched {time.time() - t0:.1f}s: {bat}")
-    steady = None
-    if steady_cases is not None:
-        t0 = time.time()
-        steady = steady_state(steady_cases, args.weights)
-        log(f"steady-state study {time.time() - t0:.1f}s: {steady}")
     out = {
         "metric": "MLUPS at 1M cells (%HBM roofline); time-to-steady-state, warm vs cold start",
-        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms / args.steps, 5), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
-        "data": "synthetic (seeded jittered-Delaunay mesh, uniform equilibrium start)",
-        "config": {"workload": workload, "cells": C, "faces": len(m.faces), "nodes": len(m.points),
-                   "scheme": "upwind", "tau": case["tau"], "dt": case["dt"],
-                   "dim_multiplier": case["dim_multiplier"], "seed": args.seed,
-                   "lattice": args.lattice,
-                   "kernel": {"k_lat_step2": "k_lat_step2 (any-lattice kernels, two cell steps per launch)",
-                              "k_lat_step": "k_lat_step (any-lattice kernels, one step per launch)",
-                              "k_step2": "k_step2 (two fused cell steps per launch)",
-                              "k_step": "k_step (one step per launch)"}[kernel],
-                   "bytes_per_lup": bpl,
-                   "residual_after": r},
-        "roofline": roof,
-        "fp32_vs_fp64": fp64,
-        "cpu_baseline": cpu,
-        "batched": bat,
-        "steady_state": steady,
-        **({"copy_probe": copy} if copy else {}),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded jittered-Delaunay meshes, uniform equilibrium start)",
     }
+    if "single" in parts:
+        fv, m, dyn, c, f, n, case = build_case(args.cells, args.seed, cache_dir=args.mesh_cache,
+                                               lattice=args.lattice)
+        C = len(m.cells)
+        opts = dict(variant=args.variant, block=args.block, reorder=args.reorder)
+        env = fv.Environment(c, f, n, precision=args.precision, **opts)
+        t0 = time.time()
+        env.init()
+        info = env.info()
+        log(f"upload {time.time() - t0:.1f}s; bandwidth {info['bandwidth']}, boundary cells "
+            f"{info['n_boundary_cells']}, device bytes {info['device_bytes'] / 1e6:.1f} MB")
+        env.step(args.warmup)
+        env.synchronize()
+        ms = env.time_steps(args.steps)
+        if getattr(env, "_lattice", False):  # the any-lattice handle has no residual kernel
+            env.step(1)
+            r = None
+            if not np.isfinite(env.cells.pdf).all():
+                raise SystemExit("non-finite pdf: the run diverged")
+        else:
+            r = env.step_residual(1)
+            if not np.isfinite(r):
+                raise SystemExit("non-finite residual: the run diverged")
+        env.close()
+        mlups = C * args.steps / (ms / 1e3) / 1e6
+        # two-step tiles unless the handle runs one step per launch (variant bit 8)
+        pairs = not (int(info["variant"]) & 8)
+        lattice_handle = getattr(env, "_lattice", False)
+        kernel = ("k_lat_step2" if pairs else "k_lat_step") if lattice_handle else ("k_step2" if pairs else "k_step")
+        updates_per_launch = 2 if pairs else 1
+        per_launch_s = ms / 1e3 / args.steps * updates_per_launch
+        bpl = bytes_per_lup(int(dyn.NUM_QUIVERS), args.precision)
+        achieved = bpl * C * updates_per_launch / per_launch_s / 1e9
+        workload = f"cylinder_channel_{C // 1000}k" if C < 999_500 else f"cylinder_channel_{round(C / 1e6)}M"
+        tr = load_traffic("cylinder_channel_1M") if abs(C - 1_000_000) < 50_000 else None
+        if tr is not None and (tr.get("kernel") != kernel or args.precision != "fp32"):
+            tr = None  # the committed counters are for another kernel or precision
+        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=(tr["bytes_per_launch"] if tr else None))
+        cpu = None if args.no_cpu_baseline else cpu_baseline(m, dyn, c, f, n)
+        fp64 = None
+        if args.precision == "fp32" and args.fp64_steps > 0:
+            # the FP32 error of the method per workload: the same kernels in FP64,
+            # after a fixed number of steps from the uniform state
+            fp64 = {workload: fp64_single(fv, c, f, n, args.fp64_steps, **opts)}
+            if not lattice_handle:
+                fp64["cavity_32x32"] = fp64_cavity(fv)
+            log(f"fp32 vs fp64: {fp64}")
+        out.update({
+            "value": round(mlups, 1), "unit": "MLUPS", "ms_per_step": round(ms / args.steps, 5),
+            "config": {"workload": workload, "cells": C, "faces": len(m.faces), "nodes": len(m.points),
+                       "scheme": "upwind", "tau": case["tau"], "dt": case["dt"],
+                       "dim_multiplier": case["dim_multiplier"], "seed": args.seed,
+                       "lattice": args.lattice,
+                       "kernel": {"k_lat_step2": "k_lat_step2 (any-lattice kernels, two cell steps per launch)",
+                                  "k_lat_step": "k_lat_step (any-lattice kernels, one step per launch)",
+                                  "k_step2": "k_step2 (two fused cell steps per launch)",
+                                  "k_step": "k_step (one step per launch)"}[kernel],
+                       "bytes_per_lup": bpl,
+                       "residual_after": r},
+            "roofline": roof,
+            "fp32_vs_fp64": fp64,
+            "cpu_baseline": cpu,
+        })
+    if batch_cases is not None:
+        t0 = time.time()
+        out["batched"] = batched(batch_cases, steps=args.batch_steps, masked_steps=args.masked_steps)
+        out["batched"]["workload"] = ("porous channels, 5-40 discs, r in [0.02,0.08], 20k-200k cells, "
+                                      "u_in in [0.02,0.1], tau in [0.55,0.9], seeds 0..")
+        log(f"batched {time.time() - t0:.1f}s: {out['batched']}")
+    if steady_cases is not None:
+        t0 = time.time()
+        out["steady_state"] = steady_state(steady_cases, args.weights, ref_cases=args.steady_ref_cases)
+        out["steady_state"]["workload"] = (
+            "unseen porous channels (seeds from %d), 3-8 discs, r in [0.02,0.06], gap 0.03, "
+            "20k-200k cells, u_in in [0.02,0.05], tau in [0.55,0.9]" % args.steady_seed0)
+        log(f"steady-state study {time.time() - t0:.1f}s: {out['steady_state']}")
+    if copy:
+        out["copy_probe"] = copy
     print(json.dumps(out), flush=True)
 
 

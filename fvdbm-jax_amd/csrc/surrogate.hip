@@ -37,7 +37,8 @@
 namespace fvdbm_nn {
 
 constexpr int HID = 64;
-constexpr int CHUNK = 256;  // rows per partial product in wgrad / colsum
+constexpr int CHUNK = 1024;  // rows per partial product in wgrad / colsum
+constexpr int RGROUP = 32;   // partials summed per thread in the first reduction pass
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -217,18 +218,22 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(const float *__restrict__ x, in
 
 // ---------------------------------------------------------------------------
 // wgrad: partial[c] = A[rows of chunk c]^T B[rows of chunk c]  ([KA][NB])
-// One block per chunk; rows staged 64 at a time in LDS as bf16; the output
-// 16x16 blocks are dealt round-robin to the four waves.
+// One block per chunk of CHUNK rows, staged 64 rows at a time in LDS as
+// bf16, transposed (feature-major, [feature][row]): an MFMA operand
+// fragment (one feature, four consecutive rows) is then one 8-byte LDS
+// read. The output 16x16 blocks are dealt round-robin to the four waves;
+// each wave's accumulators live in registers across the chunk.
 // ---------------------------------------------------------------------------
 template <int KA, int NB>
 __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
                                                const float *__restrict__ B, int M,
                                                float *__restrict__ partial) {
-  constexpr int R = 64, AP = KA + 4, BP = NB + 4;
+  constexpr int R = 64;          // rows per LDS stage
+  constexpr int RP = R + 4;      // row stride of a feature, in bf16 (8-byte aligned, odd in 8-byte units)
   constexpr int NBLK = (KA / 16) * (NB / 16);
   constexpr int PER = (NBLK + 3) / 4;
-  __shared__ unsigned short sA[R * AP];
-  __shared__ unsigned short sB[R * BP];
+  __shared__ __align__(16) unsigned short sA[KA * RP];
+  __shared__ __align__(16) unsigned short sB[NB * RP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = lane & 15, g = lane >> 4;
   const int r0 = blockIdx.x * CHUNK, r1 = min(M, r0 + CHUNK);
@@ -237,27 +242,42 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
   for (int p = 0; p < PER; ++p) acc[p] = f4{0.f, 0.f, 0.f, 0.f};
   for (int base = r0; base < r1; base += R) {
     __syncthreads();
-    for (int e = threadIdx.x; e < R * KA; e += blockDim.x) {
-      const int rr = e / KA, c = e % KA;
-      sA[rr * AP + c] = base + rr < r1 ? f2bf(A[(size_t)(base + rr) * KA + c]) : (unsigned short)0;
+    // thread e stages feature c = e % K of rows 4q .. 4q+3 (q = e / K):
+    // four coalesced loads, one 8-byte LDS store
+    for (int e = threadIdx.x; e < KA * (R / 4); e += blockDim.x) {
+      const int c = e % KA, q = e / KA;
+      unsigned short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = base + 4 * q + j;
+        v[j] = row < r1 ? f2bf(A[(size_t)row * KA + c]) : (unsigned short)0;
+      }
+      *reinterpret_cast<uint2 *>(sA + c * RP + 4 * q) =
+          make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
     }
-    for (int e = threadIdx.x; e < R * NB; e += blockDim.x) {
-      const int rr = e / NB, c = e % NB;
-      sB[rr * BP + c] = base + rr < r1 ? f2bf(B[(size_t)(base + rr) * NB + c]) : (unsigned short)0;
+    for (int e = threadIdx.x; e < NB * (R / 4); e += blockDim.x) {
+      const int c = e % NB, q = e / NB;
+      unsigned short v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = base + 4 * q + j;
+        v[j] = row < r1 ? f2bf(B[(size_t)row * NB + c]) : (unsigned short)0;
+      }
+      *reinterpret_cast<uint2 *>(sB + c * RP + 4 * q) =
+          make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
     }
     __syncthreads();
 #pragma unroll
     for (int ms = 0; ms < R / 16; ++ms) {
+      const int m = ms * 16 + 4 * g;  // this lane's four rows of the 16-row slab
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
         const int blk = wave + 4 * p;
         if (blk < NBLK) {
           const int kb = blk / (NB / 16), nb = blk % (NB / 16);
-          const int m = ms * 16 + 4 * g;
-          const s4 a = s4{(short)sA[(m + 0) * AP + kb * 16 + i], (short)sA[(m + 1) * AP + kb * 16 + i],
-                          (short)sA[(m + 2) * AP + kb * 16 + i], (short)sA[(m + 3) * AP + kb * 16 + i]};
-          const s4 b = s4{(short)sB[(m + 0) * BP + nb * 16 + i], (short)sB[(m + 1) * BP + nb * 16 + i],
-                          (short)sB[(m + 2) * BP + nb * 16 + i], (short)sB[(m + 3) * BP + nb * 16 + i]};
+          // A fragment: feature kb*16+i, rows m..m+3; B fragment: feature nb*16+i, same rows
+          const s4 a = *reinterpret_cast<const s4 *>(sA + (kb * 16 + i) * RP + m);
+          const s4 b = *reinterpret_cast<const s4 *>(sB + (nb * 16 + i) * RP + m);
           acc[p] = mfma(a, b, acc[p]);
         }
       }
@@ -301,13 +321,26 @@ __global__ void k_zero(float *__restrict__ p, int n) {
   if (e < n) p[e] = 0.f;
 }
 
-// out[e] = sum over chunks c (in order) of partial[c][e]
-__global__ void k_reduce_chunks(const float *__restrict__ partial, int nchunks, int n,
-                                float *__restrict__ out) {
+// First pass of the chunk reduction, in place: for group q (blockIdx.y),
+// partial[q * RGROUP][e] <- the sum, in chunk order, of partial[c][e] over
+// the group's chunks c = q*RGROUP .. Each thread reads only its group's
+// chunks and writes its group's first slot, so the pass is race-free.
+__global__ void k_reduce_groups(float *__restrict__ partial, int nchunks, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int c0 = blockIdx.y * RGROUP, c1 = min(nchunks, c0 + RGROUP);
+  float s = 0.f;
+  for (int c = c0; c < c1; ++c) s += partial[(size_t)c * n + e];
+  partial[(size_t)c0 * n + e] = s;
+}
+
+// Second pass: out[e] = the sum, in group order, of the group sums.
+__global__ void k_reduce_final(const float *__restrict__ partial, int nchunks, int n,
+                               float *__restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float s = 0.f;
-  for (int c = 0; c < nchunks; ++c) s += partial[(size_t)c * n + e];
+  for (int c = 0; c < nchunks; c += RGROUP) s += partial[(size_t)c * n + e];
   out[e] = s;
 }
 
@@ -394,6 +427,15 @@ template <int KA, int NB>
 static void launch_wgrad(const float *a, const float *b, int m, float *partial, hipStream_t s) {
   const int nch = (m + CHUNK - 1) / CHUNK;
   hipLaunchKernelGGL((k_wgrad<KA, NB>), dim3(nch), dim3(256), 0, s, a, b, m, partial);
+}
+
+// out [n] = sum over the nch chunk partials [nch][n] in a fixed order (in
+// groups of RGROUP chunks, then the groups); overwrites the partials
+static void launch_reduce(float *partial, int nch, int n, float *out, hipStream_t s) {
+  const int bx = (n + 255) / 256;
+  hipLaunchKernelGGL(k_reduce_groups, dim3(bx, (nch + RGROUP - 1) / RGROUP), dim3(256), 0, s,
+                     partial, nch, n);
+  hipLaunchKernelGGL(k_reduce_final, dim3(bx), dim3(256), 0, s, partial, nch, n, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -502,8 +544,7 @@ int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_
   else if (ka == 256 && nb == 64) launch_wgrad<256, 64>(a, b, mi, partial, s);
   else if (ka == 64 && nb == 16) launch_wgrad<64, 16>(a, b, mi, partial, s);
   else return nn_fail("unsupported wgrad widths");
-  const int n = ka * nb;
-  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, nch, n, out);
+  launch_reduce(partial, nch, ka * nb, out, s);
   return hip_status("wgrad launch");
 }
 
@@ -535,8 +576,7 @@ int fvdbm_nn_wgrad_noalloc(const float *a, const float *b, int64_t m, int32_t ka
   else if (ka == 256) launch_wgrad<256, 64>(a, b, mi, partial, s);
   else if (nb == 64) launch_wgrad<64, 64>(a, b, mi, partial, s);
   else launch_wgrad<64, 16>(a, b, mi, partial, s);
-  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, (int)nch, n,
-                     out);
+  launch_reduce(partial, (int)nch, n, out, s);
   return hip_status("wgrad launch");
 }
 
@@ -552,7 +592,7 @@ int fvdbm_nn_colsum(const float *a, int64_t m, int32_t n, float *partial, float 
     return hip_status("zero launch");
   }
   hipLaunchKernelGGL(k_colsum, dim3(nch), dim3(256), 0, s, a, mi, n, partial);
-  hipLaunchKernelGGL(k_reduce_chunks, dim3((n + 255) / 256), dim3(256), 0, s, partial, nch, n, out);
+  launch_reduce(partial, nch, n, out, s);
   return hip_status("colsum launch");
 }
 

@@ -14,17 +14,26 @@ its scales explicitly:
 * ``dt = CFL / (1/tau + max_cell sqrt(2) * sum_faces L')``, the explicit
   Euler positivity bound of the upwind update, rounded to float32.
 
-Two distributions:
+Three distributions:
 
 * ``WORKLOAD`` — 5-40 discs, r in [0.02,0.08], 20k-200k cells, u_in in
-  [0.02,0.1], tau in [0.55,0.9]: the batched throughput workload. Many of
-  these cases have no steady state under first-order upwind fluxes: the
-  pressure drop the velocity inlet needs grows with the numerical viscosity
-  of the scheme until the inlet density runs away. The batch controller
-  stops such cases as growing or at the step limit.
-* ``STUDY`` — 3-8 discs, r in [0.02,0.06], gap 0.03, 4k-16k cells, u_in in
-  [0.02,0.05], tau in [0.55,0.9]: cases that reach a 1e-6 residual; the
-  surrogate is trained and warm starts are measured on these.
+  [0.02,0.1], tau in [0.55,0.9]: the batched throughput workload. Most of
+  these cases have no steady state (below); ``warmstart.run_until`` stops
+  them as runaway once their mean density passes a bound.
+* ``STEADY`` — the obstacle and flow ranges of ``STUDY`` (3-8 discs, r in
+  [0.02,0.06], gap 0.03, u_in in [0.02,0.05], tau in [0.55,0.9]) at the
+  batched workload's 20k-200k cells: cases that reach a 1e-6 residual
+  (median ~110k steps); the surrogate is trained and warm starts are
+  measured on these.
+* ``STUDY`` — the same at 4k-16k cells (small cases for tests).
+
+The batched workload's velocity inlet pushes mass into a channel whose
+flow resistance grows with the obstacle count: in steady state the density
+drop across the obstacles must carry that mass flux, and when it would
+have to exceed the inlet density itself (a choked channel) no steady state
+exists and the mean density grows without bound while the velocity
+residual levels off (``warmstart.mean_density`` is the quantity a runaway
+check watches).
 
 ``node_features`` gives the surrogate's per-node inputs: x, y, signed
 distance to the nearest obstacle, a one-hot of the boundary marker (wall,
@@ -48,6 +57,8 @@ WORKLOAD = dict(count_range=(5, 40), r_range=(0.02, 0.08), gap=0.01,
                 cells_range=(20_000, 200_000), u_range=(0.02, 0.1), tau_range=(0.55, 0.9))
 STUDY = dict(count_range=(3, 8), r_range=(0.02, 0.06), gap=0.03,
              cells_range=(4_000, 16_000), u_range=(0.02, 0.05), tau_range=(0.55, 0.9))
+STEADY = dict(STUDY, cells_range=(20_000, 200_000))
+DISTRIBUTIONS = dict(workload=WORKLOAD, steady=STEADY, study=STUDY)
 
 N_FEATURES = 9
 MARKER_COLUMNS = (meshgen.WALL, meshgen.INLET, meshgen.OUTLET, meshgen.OBSTACLE)

@@ -157,12 +157,19 @@ def _interleave(u, v, ex, ey):
 
 SECTION_SHIFT = np.array([0.95, 1.06, 0.5, 0.26], np.float32)
 SECTION_SCALE = np.array([0.085, 0.116, 0.31, 0.106], np.float32)
+# optional column 15: the mesh resolution, (log(cells) - shift) / scale, the
+# same on every node of a case. First-order upwind fluxes add a numerical
+# viscosity that grows with the cell size, so the steady state of one
+# geometry depends on the resolution it is meshed at.
+RESOLUTION_COLUMN = 15
+RESOLUTION_SHIFT = 11.0   # log(60k cells)
+RESOLUTION_SCALE = 0.7
 
 
 SECTION_GRID = 2048  # x resolution of porous.section_features
 
 
-def case_inputs(case, grid=(SLABS, 1)):
+def case_inputs(case, grid=(SLABS, 1), resolution=False):
     """Scaled and padded node inputs [N + GX*GY + 1, 16] of a porous.Case
     (mesh nodes first, then the virtual grid nodes in row-major order, then
     the global node), its graph, and the number of mesh nodes.
@@ -172,7 +179,8 @@ def case_inputs(case, grid=(SLABS, 1)):
     8 neighbours (with the unit direction between centres), and the global
     node to every grid node. GY = 1 gives bands of x ("slabs"). Columns
     0-8: the node features, 9-10: virtual-node flags (grid, global), 11-14:
-    porous.section_features. Built by the library's host code
+    porous.section_features; with ``resolution``, column 15: the mesh
+    resolution (RESOLUTION_COLUMN). Built by the library's host code
     (fvdbm_nn_case_features, fvdbm_nn_case_graph), which releases the GIL,
     so threads prepare several cases at once; ``case_inputs_numpy`` is the
     same construction in NumPy."""
@@ -192,6 +200,8 @@ def case_inputs(case, grid=(SLABS, 1)):
                                   _lib.ptr(SECTION_SCALE), IN_PAD, _lib.ptr(x))
     if rc:
         raise ValueError(L.fvdbm_nn_graph_error().decode())
+    if resolution:
+        x[:, RESOLUTION_COLUMN] = (np.log(case.n_cells) - RESOLUTION_SHIFT) / RESOLUTION_SCALE
     g = case_graph(pts, porous.mesh_edges(case), grid)
     return x, g, N
 
@@ -429,11 +439,14 @@ class MeshNet:
     """The surrogate network; parameters are torch tensors on the GPU."""
 
     def __init__(self, params=None, layers=LAYERS, seed=0, target_scale=(1.0, 1.0, 1.0),
-                 grid=(SLABS, 1)):
+                 grid=(SLABS, 1), resolution=False):
         torch = _torch()
         self.layers = layers
         self.grid = tuple(int(v) for v in (params["grid"] if params is not None and "grid" in params
                                            else grid))
+        # whether the network reads the mesh-resolution input column
+        self.resolution = bool(params["resolution"]) if params is not None and "resolution" in params \
+            else bool(resolution)
         p = init_params(seed, layers) if params is None else params
         self.params = {k: torch.tensor(np.asarray(v, np.float32), device="cuda", requires_grad=True)
                        for k, v in p.items() if "." in k}
@@ -461,6 +474,7 @@ class MeshNet:
         out["target_scale"] = self.target_scale
         out["layers"] = np.int32(self.layers)
         out["grid"] = np.array(self.grid, np.int32)
+        out["resolution"] = np.int32(self.resolution)
         return out
 
     def save(self, path):
@@ -483,7 +497,7 @@ class MeshNet:
         from concurrent.futures import ThreadPoolExecutor
         out = []
         with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
-            futs = [pool.submit(case_inputs, c, self.grid) for c in cases]
+            futs = [pool.submit(case_inputs, c, self.grid, self.resolution) for c in cases]
             for c0 in range(0, len(cases), chunk):
                 ins = [f.result() for f in futs[c0:c0 + chunk]]
                 g = graph_to_device(concat_graphs([i[1] for i in ins], edge_keys=("w", "ex", "ey")))

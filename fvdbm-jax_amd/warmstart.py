@@ -15,6 +15,7 @@ import time
 import numpy as np
 
 RUNNING, CONVERGED, DIVERGED, GROWING, STOPPED = range(5)
+RUNAWAY_RHO = 2.0  # run_until stops a case whose mean density passes this
 W = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
 
 WARM_GROWTH = 100.0  # a warm case stops as growing past 100x its best residual
@@ -83,6 +84,44 @@ def solve(solver, init=None, tol=1e-6, check_every=500, max_steps=400_000, growt
         synchronize()
     return dict(steps=st["steps"].copy(), status=st["status"].copy(), residual=st["residual"].copy(),
                 fallback=fallback, wall_s=time.perf_counter() - t0)
+
+
+def mean_density(solver):
+    """Mean cell density of each case of ``solver`` (its current pdf)."""
+    rho, _ = solver.cell_fields()
+    return np.array([np.mean(solver.case_cells(k, rho), dtype=np.float64) for k in range(solver.n)])
+
+
+def run_until(solver, max_steps, check_every=2000, tol=1e-6, rho_max=RUNAWAY_RHO, chunk=10_000,
+              log=None):
+    """Step every active case until it converges (residual below ``tol`` at
+    a check), runs away or ``max_steps`` have run. A case runs away when its
+    mean density passes ``rho_max``: a velocity inlet into a channel whose
+    flow resistance would need a density drop larger than the inlet density
+    has no steady state, its density grows without bound while the velocity
+    residual levels off, so the residual alone never stops it. The density
+    is read every ``chunk`` steps. Returns the status dict of
+    ``solver.status()`` plus ``runaway`` (bool per case) and ``wall_s``."""
+    t0 = time.perf_counter()
+    runaway = np.zeros(solver.n, bool)
+    done = 0
+    st = solver.status()
+    while done < max_steps and st["active"].any():
+        n = min(chunk, max_steps - done)
+        st = solver.run(n, check_every, tol, 0.0)
+        done += n
+        high = (mean_density(solver) > rho_max) & st["active"]
+        if high.any():
+            runaway |= high
+            solver.set_active((st["active"] & ~high).astype(np.int32))
+            st = solver.status()
+        if log:
+            log(f"{done} steps: {int(st['active'].sum())} running, "
+                f"{int((st['status'] == CONVERGED).sum())} converged, {int(runaway.sum())} runaway")
+    out = dict(st)
+    out["runaway"] = runaway
+    out["wall_s"] = time.perf_counter() - t0
+    return out
 
 
 def relative_l2(a, b):

@@ -278,7 +278,7 @@ int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_
  * that forbid hipMalloc). The chunk partials go to a static 16 MiB device
  * array (one per GPU, held until the process ends), so
  * ceil(m / fvdbm_nn_chunk_rows()) * ka * nb must not exceed 4194304: at
- * most 65536 rows for (256, 64), 262144 for (64, 64) and 1048576 for
+ * most 262144 rows for (256, 64), 1048576 for (64, 64) and 4194304 for
  * (16, 64) and (64, 16); larger m fails with FVDBM_ERR_INVALID. Calls that
  * overlap on one GPU share the array and race. Results equal
  * fvdbm_nn_wgrad's bit for bit (the same kernels). */

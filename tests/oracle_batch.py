@@ -1,7 +1,7 @@
 """A CPU stand-in for BatchSolver built on the oracle, for testing the
 warm-start controller without a GPU: the same interface (reset_uniform,
 init_equilibrium_nodes, run, status, set_active, get_pdf, set_pdf,
-node_fields) and the same per-case semantics (residual of the last step of
+node_fields, cell_fields, case_cells) and the same per-case semantics (residual of the last step of
 each check interval, converged / diverged / growing / stopped)."""
 import numpy as np
 
@@ -58,6 +58,17 @@ class OracleBatch:
             rr.append(interp.node_idw(e["node_cells"], e["node_dists"], r))
             vv.append(interp.node_idw(e["node_cells"], e["node_dists"], v))
         return np.concatenate(rr), np.concatenate(vv)
+
+    def cell_fields(self):
+        rr, vv = [], []
+        for s in self.states:
+            r, v = oracle.calc_macro(s["cell_pdf"])
+            rr.append(r)
+            vv.append(v)
+        return np.concatenate(rr), np.concatenate(vv)
+
+    def case_cells(self, k, arr):
+        return arr[self.cell_off[k]:self.cell_off[k + 1]]
 
     def set_active(self, mask, reset_steps=False):
         for k, a in enumerate(np.asarray(mask).astype(bool)):

@@ -4,7 +4,8 @@ As for D2Q9, the mathematics fixes most of the lattice: the weights and
 velocities by the lattice moments (sum W = 1, sum W xi xi = c^2 I,
 isotropic fourth moment), the equilibrium by its moments (density,
 momentum, momentum flux), and the step by the uniform fixed point, mass
-balance, a brute-force two-triangle step and the mirror symmetry. The two
+balance, a brute-force two-triangle step (upwind and Lax-Wendroff) and the
+mirror symmetry. The two
 cubic equilibrium terms are pinned in ratio: sum xi feq = rho u holds only
 when the cu^3 and cu uu coefficients are in the reference's 1 : 3 ratio.
 Their common factor (the reference's 1/2 where a third-order Hermite
@@ -145,9 +146,10 @@ def check_mass_balance(mutation):
     return bool(abs(lhs + dyn.delta_t * out) < 1e-12 * np.abs(f0).sum())
 
 
-def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt):
+def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt, scheme="upwind"):
     """One step from the method's definition (see the D2Q9 version in
-    test_oracle.py), with the thirteen velocities."""
+    test_oracle.py, upwind or Lax-Wendroff face values), with the thirteen
+    velocities."""
     C = len(tris)
     cen = points[tris].mean(axis=1)
     rho = f.sum(1)
@@ -180,20 +182,27 @@ def brute_force_step(points, tris, f, node_type, node_rho, node_vel, tau, dt):
                 n = -n
             other = [o for o in owners[frozenset((a, b))] if o != c]
             fo = f[other[0]] if other else (npdf[a] + npdf[b]) - f[c]
+            mid = (points[a] + points[b]) / 2
+            dU = abs(np.dot(n, cen[c] - mid))
+            dD = abs(np.dot(n, cen[other[0]] - mid)) if other else dU
             cn = KSI @ n
-            total += L * cn * np.where(cn >= 0, f[c], fo)
+            if scheme == "upwind":
+                face = np.where(cn >= 0, f[c], fo)
+            else:
+                face = f[c] + (fo - f[c]) * (dU / (dU + dD) - cn * dt / (2 * (dU + dD)))
+            total += L * cn * face
         new[c] = f[c] + dt * ((feq[c] - f[c]) / tau - total)
     return new, npdf
 
 
-def check_brute_force(mutation):
+def check_brute_force(mutation, scheme="upwind"):
     pts = np.array([[0.0, 0.0], [1.0, 0.1], [1.1, 1.0], [-0.1, 0.9]])
     raw = meshgen.RawMesh(pts, np.array([[0, 1, 2], [0, 2, 3]], np.int32),
                           np.array([[0, 1], [1, 2], [2, 0], [2, 3], [3, 0]], np.int32),
                           np.array([1, 1, 2, 2], np.int32))
     m = build_mesher(raw)
     dyn = fv.D2Q13(tau=0.7, delta_t=0.05)
-    cells, faces, nodes = m.to_env(dyn)
+    cells, faces, nodes = m.to_env(dyn, scheme)
     nodes = m.set_vel_node(nodes, 1, np.array([0.03, -0.02]))
     nodes = m.set_rho_node(nodes, 2, 1.02)
     env = oracle_env(cells, faces, nodes, "f64")
@@ -202,9 +211,9 @@ def check_brute_force(mutation):
     f = feq_ref(np.array([1.0, 1.03]), np.array([[0.04, 0.01], [-0.02, 0.03]])) * \
         (1 + 0.05 * rng.standard_normal((2, Q)))
     st["cell_pdf"] = f.copy()
-    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f64", mutation, quivers=Q)
+    oracle.run(env, st, dyn.tau, dyn.delta_t, 1, "f64", mutation, scheme=scheme, quivers=Q)
     new, npdf = brute_force_step(m.points, m.cells, f, nodes.type.reshape(-1),
-                                 nodes.rho_f64.reshape(-1), nodes.vel_f64, dyn.tau, dyn.delta_t)
+                                 nodes.rho_f64.reshape(-1), nodes.vel_f64, dyn.tau, dyn.delta_t, scheme)
     active = nodes.type.reshape(-1) > 0
     return bool(np.allclose(st["cell_pdf"], new, rtol=0, atol=1e-14)
                 and np.allclose(st["node_pdf"][active], npdf[active], rtol=0, atol=1e-14))
@@ -235,6 +244,7 @@ CHECKS = {
     "mass_balance": check_mass_balance,
     "brute_force": check_brute_force,
     "mirror": check_mirror,
+    "lw_brute_force": lambda mut: check_brute_force(mut, "lax_wendroff"),
 }
 
 
