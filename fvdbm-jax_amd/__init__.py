@@ -19,8 +19,9 @@ from .containers import Cells, CustomArray, Faces, Nodes
 from .dynamics import D2Q9, D2Q13, Dynamics
 from .environment import Environment
 from .mesher import Mesher
+from .refiner import MeshRefiner
 
 _sys.modules.setdefault("fvdbm_jax_amd", _sys.modules[__name__])
 
 __all__ = ["Cells", "CustomArray", "D2Q9", "D2Q13", "Dynamics", "Environment", "Faces",
-           "Mesher", "Nodes", "autotune", "meshgen", "porous"]
+           "MeshRefiner", "Mesher", "Nodes", "autotune", "meshgen", "porous"]
