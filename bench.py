@@ -196,25 +196,29 @@ def fp64_cavity(fv):
 
 
 def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_cases=32,
-            fp64_cases=16, fp64_steps=20_000):
+            fp64_cases=16, fp64_steps=20_000, block=None):
     """Throughput of one batched launch over uneven porous cases (the
     batched workload's distribution), then a masked run: every case steps
     until its residual is below 1e-6 (converged), its mean density passes 2
     (runaway: a choked channel has no steady state) or ``masked_steps`` have
     run. The launch options are tuned on the first ``tune_cases`` cases
-    (every candidate checked bitwise). FP32 against FP64 on the first
-    ``fp64_cases`` cases after ``fp64_steps`` steps (per-case medians)."""
+    (every candidate checked bitwise), or fixed to tile size ``block`` with
+    two-step tiles. FP32 against FP64 on the first ``fp64_cases`` cases after
+    ``fp64_steps`` steps (per-case medians; 0 cases: skipped)."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     _lib = importlib.import_module("fvdbm-jax_amd._lib")
     import ctypes as ct
     at = importlib.import_module("fvdbm-jax_amd.autotune")
-    tuned = at.tune_batch(triples[:tune_cases],
-                          cands=at.candidates(blocks=(128, 256, 512), variants=(0, 8, 2, 4),
-                                              reorders=(4,)),
-                          steps=200, cache=None, log=log)
-    best = tuned["best"]
-    default = [r for r in tuned["results"] if (r["block"], r["variant"]) == (256, 0)]
+    if block:
+        best, default = dict(block=int(block), variant=0, reorder=4), []
+    else:
+        tuned = at.tune_batch(triples[:tune_cases],
+                              cands=at.candidates(blocks=(128, 256, 512), variants=(0, 8, 2, 4),
+                                                  reorders=(4,)),
+                              steps=200, cache=None, log=log)
+        best = tuned["best"]
+        default = [r for r in tuned["results"] if (r["block"], r["variant"]) == (256, 0)]
     t0 = time.perf_counter()
     b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
     log(f"batched handle: {len(triples)} cases uploaded in {time.perf_counter() - t0:.1f}s")
@@ -225,22 +229,41 @@ def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_ca
     _lib.check(_lib.lib().fvdbm_step_timed(b._handle(), int(steps), ct.byref(ms)), "fvdbm_step_timed")
     per = ms.value / 1e3 / steps
     mlups = cells / per / 1e6
-    b.reset_uniform()
-    b.set_active(np.ones(b.n, np.int32), reset_steps=True)
-    st = ws.run_until(b, masked_steps, check_every, 1e-6, rho_max=ws.RUNAWAY_RHO, chunk=10_000,
-                      log=lambda m: log("masked: " + m))
+    masked = None
+    if masked_steps > 0:
+        b.reset_uniform()
+        b.set_active(np.ones(b.n, np.int32), reset_steps=True)
+        st = ws.run_until(b, masked_steps, check_every, 1e-6, rho_max=ws.RUNAWAY_RHO, chunk=10_000,
+                          log=lambda m: log("masked: " + m))
+        masked = _masked_summary(st, masked_steps, check_every)
     b.close()
+    fp64 = _batch_fp64(triples[:fp64_cases], best, fp64_steps) if fp64_cases > 0 else None
+    return dict(cases=len(triples), cells=cells, padded_cells=int(info["n_cells_padded"]), steps=steps,
+                launch_classes=int(info["launch_classes"]),
+                tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"],
+                           on_cases=0 if block else min(tune_cases, len(triples))),
+                default_us_per_step=default[0]["us_per_step"] if default else None,
+                us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
+                hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
+                masked=masked, fp32_vs_fp64=fp64)
+
+
+def _masked_summary(st, masked_steps, check_every):
+    ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     conv = st["status"] == ws.CONVERGED
     running = st["active"]
-    masked = dict(max_steps=masked_steps, check_every=check_every, runaway_mean_rho=ws.RUNAWAY_RHO,
-                  wall_s=round(st["wall_s"], 2), converged=int(conv.sum()),
-                  runaway=int(st["runaway"].sum()), running=int(running.sum()),
-                  diverged=int((st["status"] == ws.DIVERGED).sum()),
-                  steps_converged_median=float(np.median(st["steps"][conv])) if conv.any() else None,
-                  steps_runaway_median=float(np.median(st["steps"][st["runaway"]]))
-                  if st["runaway"].any() else None)
-    # FP32 against FP64 at a fixed horizon, per case
-    sub = triples[:fp64_cases]
+    return dict(max_steps=masked_steps, check_every=check_every, runaway_mean_rho=ws.RUNAWAY_RHO,
+                wall_s=round(st["wall_s"], 2), converged=int(conv.sum()),
+                runaway=int(st["runaway"].sum()), running=int(running.sum()),
+                diverged=int((st["status"] == ws.DIVERGED).sum()),
+                steps_converged_median=float(np.median(st["steps"][conv])) if conv.any() else None,
+                steps_runaway_median=float(np.median(st["steps"][st["runaway"]]))
+                if st["runaway"].any() else None)
+
+
+def _batch_fp64(sub, best, fp64_steps):
+    """FP32 against FP64 after a fixed number of steps, per case."""
+    batch = importlib.import_module("fvdbm-jax_amd.batch")
     res = {}
     for prec in ("fp32", "fp64"):
         bb = batch.BatchSolver(sub, precision=prec, block=best["block"], variant=best["variant"],
@@ -250,18 +273,10 @@ def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_ca
         res[prec] = [(bb.case_cells(k, r), bb.case_cells(k, u)) for k in range(bb.n)]
         bb.close()
     errs = [_moment_errors(*res["fp32"][k], *res["fp64"][k]) for k in range(len(sub))]
-    fp64 = dict(cases=len(sub), steps=fp64_steps,
+    return dict(cases=len(sub), steps=fp64_steps,
                 rel_l2_u_median=float(np.median([e["rel_l2_u"] for e in errs])),
                 rel_l2_u_max=float(np.max([e["rel_l2_u"] for e in errs])),
                 rel_l2_rho_median=float(np.median([e["rel_l2_rho"] for e in errs])))
-    return dict(cases=len(triples), cells=cells, padded_cells=int(info["n_cells_padded"]), steps=steps,
-                launch_classes=int(info["launch_classes"]),
-                tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"],
-                           on_cases=min(tune_cases, len(triples))),
-                default_us_per_step=default[0]["us_per_step"] if default else None,
-                us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
-                hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),
-                masked=masked, fp32_vs_fp64=fp64)
 
 
 def steady_state(cases, weights, tol=1e-6, check_every=1000, max_steps=800_000, ref_cases=16,
@@ -381,6 +396,13 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000, help="timed steps of the batched launch")
     ap.add_argument("--masked-steps", type=int, default=200_000,
                     help="step cap of the batched masked run (0: skip it)")
+    ap.add_argument("--mesh-workers", type=int, default=16,
+                    help="processes meshing the porous cases (0: in this process, e.g. under a "
+                         "profiler that must not see forked children)")
+    ap.add_argument("--batch-block", type=int, default=0,
+                    help="fix the batched tile size (two-step tiles) instead of tuning it")
+    ap.add_argument("--batch-fp64-cases", type=int, default=16,
+                    help="batched cases of the FP32-against-FP64 comparison (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); "
@@ -399,14 +421,17 @@ def main():
         ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
         t0 = time.time()
         steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases),
-                                      dist=fvp.porous.STEADY)
+                                      workers=max(1, args.mesh_workers), dist=fvp.porous.STEADY)
         log(f"steady-state study: {len(steady_cases)} cases meshed in {time.time() - t0:.1f}s")
     batch_cases = None
     if "batched" in parts and args.batch_cases > 0:
         import multiprocessing as mp
         t0 = time.time()
-        with mp.get_context("fork").Pool(16) as pool:
-            batch_cases = pool.map(_build_workload, range(args.batch_cases))
+        if args.mesh_workers > 0:
+            with mp.get_context("fork").Pool(args.mesh_workers) as pool:
+                batch_cases = pool.map(_build_workload, range(args.batch_cases))
+        else:
+            batch_cases = [_build_workload(k) for k in range(args.batch_cases)]
         log(f"batched workload: {len(batch_cases)} cases meshed in {time.time() - t0:.1f}s")
 
     copy = None
@@ -494,7 +519,8 @@ def main():
         })
     if batch_cases is not None:
         t0 = time.time()
-        out["batched"] = batched(batch_cases, steps=args.batch_steps, masked_steps=args.masked_steps)
+        out["batched"] = batched(batch_cases, steps=args.batch_steps, masked_steps=args.masked_steps,
+                                 block=args.batch_block, fp64_cases=args.batch_fp64_cases)
         out["batched"]["workload"] = ("porous channels, 5-40 discs, r in [0.02,0.08], 20k-200k cells, "
                                       "u_in in [0.02,0.1], tau in [0.55,0.9], seeds 0..")
         log(f"batched {time.time() - t0:.1f}s: {out['batched']}")

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -98,11 +99,13 @@ __device__ __forceinline__ void macro(const T (&f)[QN], T &rho, T &ux, T &uy) {
   T r = T(0);
 #pragma unroll
   for (int q = 0; q < QN; ++q) r += f[q];
+  // terms with a zero ksi component add a zero to an accumulator that is
+  // never -0 (it starts at +0): left out, exact for finite f
   T mx = T(0), my = T(0);
 #pragma unroll
   for (int q = 0; q < QN; ++q) {
-    mx += T(kx(q)) * f[q];
-    my += T(ky(q)) * f[q];
+    if (kx(q) != 0) mx += T(kx(q)) * f[q];
+    if (ky(q) != 0) my += T(ky(q)) * f[q];
   }
   rho = r;
   ux = mx / r;
@@ -821,8 +824,8 @@ struct Handle {
        *xentry_pair_off = nullptr, *xpair_off = nullptr, *xpair_cell = nullptr, *xpair_w = nullptr,
        *entry_pair_off = nullptr, *tile_pair_off = nullptr, *pair_cell = nullptr,
        *pair_slot = nullptr, *pair_w = nullptr;
-  hipGraph_t graph2[2] = {nullptr, nullptr};
-  hipGraphExec_t gexec2[2] = {nullptr, nullptr};
+  // captured runs of two-step launches, keyed by (starting parity, launches)
+  std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs2;
   // two-step handles: after a run that ended in a two-step launch
   // (prev_two_back), f[cur ^ 1] holds the pdf two steps back; the pdf at the
   // start of the last step is one step from it, computed into fmid only
@@ -842,8 +845,10 @@ struct Handle {
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
-      if (gexec2[i]) (void)hipGraphExecDestroy(gexec2[i]);
-      if (graph2[i]) (void)hipGraphDestroy(graph2[i]);
+    }
+    for (auto &kv : graphs2) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
     }
     for (void *b : bufs) (void)hipFree(b);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -984,12 +989,18 @@ static int pair_any(Handle &h, int src) {
   });
 }
 
-// graph_steps / 2 two-step launches starting from `parity`
-static int ensure_graph2(Handle &h, int parity) {
-  if (h.gexec2[parity]) return FVDBM_OK;
+// a captured run of `launches` two-step launches starting from `parity`
+// (cached per (parity, launches)); *out is its executable graph
+static int ensure_graph2(Handle &h, int parity, int launches, hipGraphExec_t *out) {
+  const auto key = std::make_pair(parity, launches);
+  auto it = h.graphs2.find(key);
+  if (it != h.graphs2.end()) {
+    *out = it->second.second;
+    return FVDBM_OK;
+  }
   LHIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
   int src = parity, rc = FVDBM_OK;
-  for (int s = 0; s < h.graph_steps / 2 && rc == FVDBM_OK; ++s) {
+  for (int s = 0; s < launches && rc == FVDBM_OK; ++s) {
     rc = pair_any(h, src);
     src ^= 1;
   }
@@ -1000,8 +1011,14 @@ static int ensure_graph2(Handle &h, int parity) {
     return rc;
   }
   if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  h.graph2[parity] = g;
-  LHIP_TRY(hipGraphInstantiate(&h.gexec2[parity], g, nullptr, nullptr, 0));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(FVDBM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  h.graphs2[key] = std::make_pair(g, x);
+  *out = x;
   return FVDBM_OK;
 }
 
@@ -1025,41 +1042,56 @@ static int ensure_graph(Handle &h, int parity) {
   return FVDBM_OK;
 }
 
-static int run_steps(Handle &h, int64_t n) {
+// n steps. Upwind handles with two-step tiles run two-step launches only
+// (an odd count starts with one single step): captured runs of
+// graph_steps / 2 launches, then one captured run for the rest. With dry,
+// nothing is launched and only the graphs such a run replays are built.
+static int run_steps(Handle &h, int64_t n, bool dry = false) {
   if (n <= 0) return FVDBM_OK;
   if (h.pairs && n >= 2) {
-    // two-step launches only; an odd count starts with one single step.
-    // f[cur ^ 1] then holds the pdf two steps back (see prev_two_back)
+    int cur = h.cur;
     if (n & 1) {
-      int rc = step_any(h, h.cur);
-      if (rc) return rc;
-      h.cur ^= 1;
-      ++h.steps;
+      if (!dry) {
+        int rc = step_any(h, cur);
+        if (rc) return rc;
+        ++h.steps;
+      }
+      cur ^= 1;
       --n;
     }
-    int64_t rest = n - 2;
-    if (h.graph_steps >= 2) {
-      while (rest >= h.graph_steps) {
-        int rc = ensure_graph2(h, h.cur);
-        if (rc) return rc;
-        LHIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
-        h.cur ^= (h.graph_steps / 2) & 1;
-        rest -= h.graph_steps;
-        h.steps += h.graph_steps;
-      }
-    }
-    for (; rest >= 0; rest -= 2) {
-      int rc = pair_any(h, h.cur);
+    int64_t launches = n / 2;
+    const int per = h.graph_steps >= 2 ? h.graph_steps / 2 : 0;
+    hipGraphExec_t x = nullptr;
+    while (per > 0 && launches > 0) {
+      const int nl = (int)std::min<int64_t>(launches, per);
+      if (nl < 2) break;
+      int rc = ensure_graph2(h, cur, nl, &x);
       if (rc) return rc;
-      h.cur ^= 1;
-      h.steps += 2;
+      if (!dry) {
+        LHIP_TRY(hipGraphLaunch(x, h.stream));
+        h.steps += 2 * (int64_t)nl;
+      }
+      cur ^= nl & 1;
+      launches -= nl;
     }
+    for (; launches > 0; --launches) {
+      if (!dry) {
+        int rc = pair_any(h, cur);
+        if (rc) return rc;
+        h.steps += 2;
+      }
+      cur ^= 1;
+    }
+    if (dry) return FVDBM_OK;
+    // f[cur ^ 1] holds the pdf two steps back (see prev_two_back)
+    h.cur = cur;
     h.prev_two_back = true;
     h.mid_ready = false;
     h.has_prev = true;
     h.observed = false;
     return FVDBM_OK;
   }
+  if (dry) return h.graph_steps > 0 && n >= h.graph_steps ? ensure_graph(h, h.cur) : FVDBM_OK;
   if (h.graph_steps > 0) {
     while (n >= h.graph_steps) {
       int rc = ensure_graph(h, h.cur);
@@ -1490,11 +1522,8 @@ int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms) {
   Handle *h = static_cast<Handle *>(handle);
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
-  if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
-    // outside the timed region; both parities, since an odd run of
-    // two-step launches starts with one step
-    int rc = h->pairs ? ensure_graph2(*h, h->cur) : ensure_graph(*h, h->cur);
-    if (!rc && h->pairs) rc = ensure_graph2(*h, h->cur ^ 1);
+  {
+    int rc = run_steps(*h, nsteps, true);  // the graphs this run replays, outside the timed region
     if (rc) return rc;
   }
   LHIP_TRY(hipEventRecord(h->ev0, h->stream));

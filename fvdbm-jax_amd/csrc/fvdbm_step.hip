@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -96,7 +97,11 @@ static Lattice<T> make_lattice(double tau, double dt) {
 // device math: one function per reference formula, same operation order
 // ----------------------------------------------------------------------------
 
-// rho = sum_q f_q ; u = (sum_q ksi_q f_q) / rho, sums 0-initialised in q order
+// rho = sum_q f_q ; u = (sum_q ksi_q f_q) / rho, sums 0-initialised in q order.
+// The terms with ksi component 0 are left out: 0 * f is a zero of either
+// sign, and an accumulator that starts at +0 is never -0 (x + y is -0 only
+// for two -0 operands), so adding such a zero changes nothing for finite f.
+// ksi components of +-1 give +-f exactly.
 template <typename T>
 __device__ __forceinline__ void macro(const T (&f)[Q], T &rho, T &ux, T &uy) {
   T r = T(0);
@@ -105,8 +110,8 @@ __device__ __forceinline__ void macro(const T (&f)[Q], T &rho, T &ux, T &uy) {
   T mx = T(0), my = T(0);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    mx += T(kx(q)) * f[q];
-    my += T(ky(q)) * f[q];
+    if (kx(q) != 0) mx += T(kx(q)) * f[q];
+    if (ky(q) != 0) my += T(ky(q)) * f[q];
   }
   rho = r;
   ux = mx / r;
@@ -1300,6 +1305,10 @@ struct Handle {
   // limit its occupancy) and the others in a second one; 0 = one launch over
   // block_tile with lds2
   int npair_class = 0;
+  // with two classes the second (few tiles, larger LDS) runs concurrently on
+  // stream2, forked from and joined back into the handle's stream (evf, evj)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t evf = nullptr, evj = nullptr;
   void *pair_bt[2] = {nullptr, nullptr};
   int pair_grid[2] = {0, 0};
   size_t pair_lds[2] = {0, 0};
@@ -1309,8 +1318,8 @@ struct Handle {
        *rg_geo = nullptr, *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr,
        *xnodes = nullptr, *xentry_pair_off = nullptr, *xpair_off = nullptr,
        *xpair_cell = nullptr, *xpair_w = nullptr, *pair_slot = nullptr;
-  hipGraphExec_t gexec2[2] = {nullptr, nullptr};
-  hipGraph_t graph2[2] = {nullptr, nullptr};
+  // captured runs of two-step launches, keyed by (starting parity, launches)
+  std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs2;
   // two-step handles: after a run that ended in a two-step launch
   // (prev_two_back), f[cur ^ 1] holds the pdf two steps back; the pdf at the
   // start of the last step is one k_step from it, computed into fmid only
@@ -1351,8 +1360,10 @@ struct Handle {
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
-      if (gexec2[i]) (void)hipGraphExecDestroy(gexec2[i]);
-      if (graph2[i]) (void)hipGraphDestroy(graph2[i]);
+    }
+    for (auto &kv : graphs2) {
+      if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+      if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
     }
     if (bexec) (void)hipGraphExecDestroy(bexec);
     if (sexec) (void)hipGraphExecDestroy(sexec);
@@ -1362,6 +1373,9 @@ struct Handle {
       if (b.p) (void)hipFree(b.p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (evf) (void)hipEventDestroy(evf);
+    if (evj) (void)hipEventDestroy(evj);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1517,14 +1531,24 @@ static int launch_pair_typed(Handle &h, int src) {
     return FVDBM_OK;
   }
   // the classes partition the tiles, and a tile reads only fsrc and writes
-  // only its own cells of fdst: two launches are one two-step launch
-  for (int k = 0; k < h.npair_class; ++k) {
+  // only its own cells of fdst: two launches are one two-step launch, and
+  // they may run at the same time. The heavy class goes first, on stream2,
+  // so its few long tiles overlap the light class instead of trailing it.
+  if (h.stream2) {
+    HIP_TRY(hipEventRecord(h.evf, h.stream));
+    HIP_TRY(hipStreamWaitEvent(h.stream2, h.evf, 0));
+  }
+  for (int k = h.npair_class - 1; k >= 0; --k) {
     a.block_tile = static_cast<const int32_t *>(h.pair_bt[k]);
     p.max_xnodes = h.pair_xnodes[k];
     p.max_xpairs = h.pair_xpairs[k];
-    hipLaunchKernelGGL((k_step2<T>), dim3(h.pair_grid[k]), dim3(h.block), h.pair_lds[k],
-                       h.stream, a, p);
+    hipStream_t s = (k == 1 && h.stream2) ? h.stream2 : h.stream;
+    hipLaunchKernelGGL((k_step2<T>), dim3(h.pair_grid[k]), dim3(h.block), h.pair_lds[k], s, a, p);
     HIP_TRY(hipGetLastError());
+  }
+  if (h.stream2) {
+    HIP_TRY(hipEventRecord(h.evj, h.stream2));
+    HIP_TRY(hipStreamWaitEvent(h.stream, h.evj, 0));
   }
   return FVDBM_OK;
 }
@@ -1638,13 +1662,19 @@ static int ensure_graph(Handle &h, int parity) {
   return FVDBM_OK;
 }
 
-// Two-step tiles: graph_steps / 2 pair launches starting from `parity`.
-static int ensure_graph2(Handle &h, int parity) {
-  if (h.gexec2[parity]) return FVDBM_OK;
+// Two-step tiles: a captured run of `launches` pair launches starting from
+// `parity` (cached per (parity, launches)); *out is its executable graph.
+static int ensure_graph2(Handle &h, int parity, int launches, hipGraphExec_t *out) {
+  const auto key = std::make_pair(parity, launches);
+  auto it = h.graphs2.find(key);
+  if (it != h.graphs2.end()) {
+    *out = it->second.second;
+    return FVDBM_OK;
+  }
   HIP_TRY(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
   int src = parity;
   int rc = FVDBM_OK;
-  for (int s = 0; s < h.graph_steps / 2 && rc == FVDBM_OK; ++s) {
+  for (int s = 0; s < launches && rc == FVDBM_OK; ++s) {
     rc = launch_pair(h, src);
     src ^= 1;
   }
@@ -1655,8 +1685,46 @@ static int ensure_graph2(Handle &h, int parity) {
     return rc;
   }
   if (e != hipSuccess) return fail(FVDBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  h.graph2[parity] = g;
-  HIP_TRY(hipGraphInstantiate(&h.gexec2[parity], g, nullptr, nullptr, 0));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(FVDBM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  h.graphs2[key] = std::make_pair(g, x);
+  *out = x;
+  return FVDBM_OK;
+}
+
+// `launches` pair launches from f[h.cur]: captured runs of graph_steps / 2
+// launches, then one captured run for the rest (a single launch goes
+// direct). With dry, nothing is launched and only the graphs such a run
+// would replay are built (so a timed run captures nothing).
+static int run_pair_launches(Handle &h, int64_t launches, bool dry) {
+  int cur = h.cur;
+  const int per = h.graph_steps >= 2 ? h.graph_steps / 2 : 0;
+  hipGraphExec_t x = nullptr;
+  while (per > 0 && launches > 0) {
+    const int nl = (int)std::min<int64_t>(launches, per);
+    if (nl < 2) break;
+    int rc = ensure_graph2(h, cur, nl, &x);
+    if (rc) return rc;
+    if (!dry) {
+      HIP_TRY(hipGraphLaunch(x, h.stream));
+      h.steps += 2 * (int64_t)nl;
+    }
+    cur ^= nl & 1;
+    launches -= nl;
+  }
+  for (; launches > 0; --launches) {
+    if (!dry) {
+      int rc = launch_pair(h, cur);
+      if (rc) return rc;
+      h.steps += 2;
+    }
+    cur ^= 1;
+  }
+  if (!dry) h.cur = cur;
   return FVDBM_OK;
 }
 
@@ -1699,64 +1767,45 @@ static int run_batch_steps(Handle &h, int64_t n, bool resid_last) {
   return FVDBM_OK;
 }
 
-static int run_steps(Handle &h, int64_t n, bool resid_last) {
+// n steps (the residual on the last when asked). With dry (no residual),
+// only the graphs such a run replays are built.
+static int run_steps(Handle &h, int64_t n, bool resid_last, bool dry = false) {
   if (n <= 0) return FVDBM_OK;
-  if (h.batch) return run_batch_steps(h, n, resid_last);
+  if (h.batch) return dry ? FVDBM_OK : run_batch_steps(h, n, resid_last);
   if (h.pairs && !resid_last && n >= 2) {
     // two-step launches only; an odd count starts with one single step.
     // f[cur ^ 1] then holds the pdf two steps back (see prev_two_back)
+    const int cur0 = h.cur;
     if (n & 1) {
-      int rc = launch_step_any(h, h.cur, false);
-      if (rc) return rc;
+      if (!dry) {
+        int rc = launch_step_any(h, h.cur, false);
+        if (rc) return rc;
+        ++h.steps;
+      }
       h.cur ^= 1;
-      ++h.steps;
       --n;
     }
-    int64_t rest = n - 2;
-    if (h.graph_steps >= 2) {
-      while (rest >= h.graph_steps) {
-        int rc = ensure_graph2(h, h.cur);
-        if (rc) return rc;
-        HIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
-        h.cur ^= (h.graph_steps / 2) & 1;
-        rest -= h.graph_steps;
-        h.steps += h.graph_steps;
-      }
+    int rc = run_pair_launches(h, n / 2, dry);
+    if (dry) {
+      h.cur = cur0;
+      return rc;
     }
-    for (; rest >= 0; rest -= 2) {
-      int rc = launch_pair(h, h.cur);
-      if (rc) return rc;
-      h.cur ^= 1;
-      h.steps += 2;
-    }
+    if (rc) return rc;
     h.prev_two_back = true;
     h.mid_ready = false;
     h.has_prev = true;
     h.observed = false;
     return FVDBM_OK;
   }
+  if (dry) return FVDBM_OK;
   if (h.pairs && n >= 3) {
     // residual runs: pairs of steps for all but the last one or two, so the
     // last (residual) step is a single launch and f[cur ^ 1] keeps the pdf
     // at its start
-    int64_t rest = n - 1;
-    if (h.graph_steps >= 2) {
-      while (rest >= h.graph_steps) {
-        int rc = ensure_graph2(h, h.cur);
-        if (rc) return rc;
-        HIP_TRY(hipGraphLaunch(h.gexec2[h.cur], h.stream));
-        h.cur ^= (h.graph_steps / 2) & 1;
-        rest -= h.graph_steps;
-        h.steps += h.graph_steps;
-      }
-    }
-    for (; rest >= 2; rest -= 2) {
-      int rc = launch_pair(h, h.cur);
-      if (rc) return rc;
-      h.cur ^= 1;
-      h.steps += 2;
-    }
-    n = rest + 1;  // the remaining single steps, the last included
+    const int64_t launches = (n - 1) / 2;
+    int rc = run_pair_launches(h, launches, false);
+    if (rc) return rc;
+    n -= 2 * launches;  // the remaining single steps, the last included
   }
   int64_t plain = resid_last ? n - 1 : n;
   if (h.graph_steps > 0 && !h.batch) {
@@ -1876,6 +1925,9 @@ static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
     h.pair_xnodes[k] = cls[k].rows;
     h.pair_xpairs[k] = cls[k].pairs;
   }
+  HIP_TRY(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&h.evf, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&h.evj, hipEventDisableTiming));
   h.npair_class = 2;
   return FVDBM_OK;
 }
@@ -2796,12 +2848,13 @@ int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms) {
   Handle *h = static_cast<Handle *>(handle);
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");
   // build graphs outside the timed region
-  if (h->graph_steps > 0 && nsteps >= h->graph_steps) {
-    // both parities: an odd run of two-step launches starts with one step
-    int rc = h->batch  ? fvdbm::ensure_batch_step_graph(*h)
-             : h->pairs ? fvdbm::ensure_graph2(*h, h->cur)
-                        : fvdbm::ensure_graph(*h, h->cur);
-    if (!rc && h->pairs && !h->batch) rc = fvdbm::ensure_graph2(*h, h->cur ^ 1);
+  {
+    // the graphs this run replays, captured outside the timed region
+    int rc = FVDBM_OK;
+    if (h->graph_steps > 0 && nsteps >= h->graph_steps)
+      rc = h->batch ? fvdbm::ensure_batch_step_graph(*h)
+                    : h->pairs ? FVDBM_OK : fvdbm::ensure_graph(*h, h->cur);
+    if (!rc && h->pairs && !h->batch) rc = fvdbm::run_steps(*h, nsteps, false, true);
     if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));

@@ -2,8 +2,10 @@
 interpolation restarts a case in a fraction of the cold steps and stops at a
 field within the float32 fixed-point band of the cold one; corrupted
 predictions fall back to the cold start, which then reproduces the cold
-solve bitwise."""
+solve bitwise; the shipped network's predictions on unseen cases take fewer
+steps to the residual than cold starts."""
 import importlib
+import os
 
 import numpy as np
 import pytest
@@ -41,3 +43,32 @@ def test_warm_and_fallback(gpu):
         assert warm["steps"][k] == cold["steps"][k]
         assert np.array_equal(b.case_cells(k, pdf_warm), b.case_cells(k, pdf_cold))
     b.close()
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fvdbm-jax_amd",
+                       "data", "surrogate_porous.npz")
+
+
+def test_shipped_network_gains_over_cold(gpu):
+    """The shipped surrogate (trained on steady states the solver generated
+    for seeds below 2000) on 8 unseen cases of the steady-state
+    distribution (3-8 discs; seeds from 3,000,000; 20k-31k cells): no
+    prediction needs the cold fallback, every warm solve converges to the
+    1e-6 residual, and the warm solves take fewer steps than the cold ones,
+    case by case in the median (ratio above 1.3) and in total."""
+    sg = importlib.import_module("fvdbm-jax_amd.surrogate")
+    cases = [fv.porous.make_case(3_000_000 + k, dist=fv.porous.STEADY, target_cells=20_000 + 1_500 * k)
+             for k in range(8)]
+    net = sg.MeshNet.load(WEIGHTS)
+    pred = net.predict(cases)
+    init = (np.concatenate([p[0] for p in pred]), np.concatenate([p[1] for p in pred]))
+    b = _solver(cases)
+    cold = ws.solve(b, None, tol=TOL, check_every=1000, max_steps=600_000)
+    warm = ws.solve(b, init, tol=TOL, check_every=1000, max_steps=600_000)
+    b.close()
+    print("cold steps", cold["steps"].tolist(), "warm steps", warm["steps"].tolist(), flush=True)
+    assert (cold["status"] == ws.CONVERGED).all(), cold
+    assert (warm["status"] == ws.CONVERGED).all() and not warm["fallback"].any(), warm
+    ratio = cold["steps"] / warm["steps"]
+    assert np.median(ratio) > 1.3, ratio
+    assert warm["steps"].sum() < cold["steps"].sum()
