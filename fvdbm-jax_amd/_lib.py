@@ -73,6 +73,7 @@ _SIGNATURES = {
     "fvdbm_get_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),
     "fvdbm_step": (ct.c_int, [ct.c_void_p, ct.c_int64]),
     "fvdbm_step_residual": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
+    "fvdbm_prepare": (ct.c_int, [ct.c_void_p, ct.c_int64]),
     "fvdbm_step_timed": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
     "fvdbm_synchronize": (ct.c_int, [ct.c_void_p]),
     "fvdbm_get_permutation": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
@@ -82,6 +83,7 @@ _SIGNATURES = {
     "fvdbm_lattice_destroy": (ct.c_int, [ct.c_void_p]),
     "fvdbm_lattice_init_uniform": (ct.c_int, [ct.c_void_p]),
     "fvdbm_lattice_step": (ct.c_int, [ct.c_void_p, ct.c_int64]),
+    "fvdbm_lattice_prepare": (ct.c_int, [ct.c_void_p, ct.c_int64]),
     "fvdbm_lattice_step_timed": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double)]),
     "fvdbm_lattice_synchronize": (ct.c_int, [ct.c_void_p]),
     "fvdbm_lattice_get_field": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p]),

@@ -1517,6 +1517,21 @@ int fvdbm_lattice_step(void *handle, int64_t nsteps) {
   return run_steps(*h, nsteps);
 }
 
+int fvdbm_lattice_prepare(void *handle, int64_t nsteps) {
+  using namespace fvdbm_lattice;
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
+  const int cur = h->cur;  // from either buffer parity
+  int rc = FVDBM_OK;
+  for (int par = 0; par < 2 && rc == FVDBM_OK; ++par) {
+    h->cur = par;
+    rc = run_steps(*h, nsteps, true);
+  }
+  h->cur = cur;
+  return rc;
+}
+
 int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms) {
   using namespace fvdbm_lattice;
   Handle *h = static_cast<Handle *>(handle);

@@ -2844,6 +2844,24 @@ int fvdbm_step_residual(void *handle, int64_t nsteps, double *residual) {
   return FVDBM_OK;
 }
 
+int fvdbm_prepare(void *handle, int64_t nsteps) {
+  Handle *h = static_cast<Handle *>(handle);
+  if (!h) return fail(FVDBM_ERR_INVALID, "null handle");
+  if (nsteps < 0) return fail(FVDBM_ERR_INVALID, "nsteps < 0");
+  if (h->batch) return h->graph_steps > 0 ? fvdbm::ensure_batch_step_graph(*h) : FVDBM_OK;
+  // from either buffer parity: the steps before the run decide which
+  const int cur = h->cur;
+  int rc = FVDBM_OK;
+  for (int par = 0; par < 2 && rc == FVDBM_OK; ++par) {
+    h->cur = par;
+    rc = h->pairs ? fvdbm::run_steps(*h, nsteps, false, true)
+                  : (h->graph_steps > 0 && nsteps >= h->graph_steps ? fvdbm::ensure_graph(*h, par)
+                                                                     : FVDBM_OK);
+  }
+  h->cur = cur;
+  return rc;
+}
+
 int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms) {
   Handle *h = static_cast<Handle *>(handle);
   if (!h || !ms) return fail(FVDBM_ERR_INVALID, "null argument");

@@ -44,7 +44,7 @@ _FIELDS = {
 # entry points of the two handle kinds
 _TUNED_API = dict(destroy="fvdbm_destroy", set="fvdbm_set_field", get="fvdbm_get_field",
                   step="fvdbm_step", timed="fvdbm_step_timed", sync="fvdbm_synchronize",
-                  info="fvdbm_info_get", perm="fvdbm_get_permutation")
+                  info="fvdbm_info_get", perm="fvdbm_get_permutation", prepare="fvdbm_prepare")
 _LATTICE_API = {k: v.replace("fvdbm_", "fvdbm_lattice_", 1) for k, v in _TUNED_API.items()}
 _SETTABLE = {("cells", "pdf"), ("nodes", "pdf"), ("nodes", "rho"), ("nodes", "vel")}
 
@@ -243,8 +243,16 @@ class Environment(_lib.FieldsMixin):
                 break
         return steps, hist
 
+    def prepare(self, nsteps):
+        """Capture, on the host, the graphs a later run of ``nsteps`` steps
+        replays (nothing runs), so that run starts without capturing."""
+        self._call("prepare", int(nsteps))
+
     def time_steps(self, nsteps):
-        """Run ``nsteps`` between HIP events; returns elapsed milliseconds."""
+        """Run ``nsteps`` between HIP events; returns elapsed milliseconds.
+        Steps already queued (``step`` is asynchronous) run before the first
+        event, so a warm-up queued just before keeps the device busy while
+        this run is submitted."""
         ms = ct.c_double()
         self._call("timed", int(nsteps), ct.byref(ms))
         return float(ms.value)

@@ -157,8 +157,13 @@ int fvdbm_step(void *handle, int64_t nsteps);
 /* Advance nsteps steps and return the residual of the last one,
  * R = ||u^{n+1} - u^n||_2 / ||u^{n+1}||_2 over all cells (synchronous). */
 int fvdbm_step_residual(void *handle, int64_t nsteps, double *residual);
+/* Capture the hipGraphs a later run of nsteps steps replays (from either
+ * buffer parity), on the host: nothing runs and the state is unchanged, so
+ * that run (fvdbm_step or fvdbm_step_timed) launches without capturing. */
+int fvdbm_prepare(void *handle, int64_t nsteps);
 /* Advance nsteps steps between two HIP events on the handle's stream and
- * return the elapsed milliseconds (synchronous). */
+ * return the elapsed milliseconds (synchronous). Work already queued on
+ * the stream runs before the first event. */
 int fvdbm_step_timed(void *handle, int64_t nsteps, double *ms);
 int fvdbm_synchronize(void *handle);
 
@@ -180,6 +185,7 @@ int fvdbm_lattice_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *params
 int fvdbm_lattice_destroy(void *handle);
 int fvdbm_lattice_init_uniform(void *handle);
 int fvdbm_lattice_step(void *handle, int64_t nsteps);
+int fvdbm_lattice_prepare(void *handle, int64_t nsteps);
 int fvdbm_lattice_step_timed(void *handle, int64_t nsteps, double *ms);
 int fvdbm_lattice_synchronize(void *handle);
 int fvdbm_lattice_get_field(void *handle, int32_t field, void *dst);

@@ -236,6 +236,24 @@ def test_boundary_values_set_after_a_two_step_run(gpu):
     _assert_fields_equal(env, st, what="3 steps with the new values")
 
 
+def test_prepare_changes_nothing(gpu):
+    """Capturing a run's graphs ahead (prepare) leaves the state as it was,
+    and the run then gives the oracle's result, from either parity."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=6)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    env = _gpu_env(cells, faces, nodes)
+    env.step(3)
+    before = env.cells.pdf
+    env.prepare(20)
+    assert np.array_equal(env.cells.pdf, before) and env.info()["steps_done"] == 3
+    env.step(5)  # odd: the timed run below starts from the other parity
+    ms = env.time_steps(20)
+    assert ms > 0
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 28, "f32")
+    _assert_fields_equal(env, st, what="prepared run")
+
+
 def test_every_cell_written_and_deterministic(gpu):
     """Two runs agree bitwise and every cell holds a finite pdf."""
     m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=2)
