@@ -578,13 +578,17 @@ def concat_device(samples):
             torch.cat([s["mask"] for s in samples]))
 
 
-def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final=1e-5):
-    """Adam on the mean squared error over mesh nodes, learning rate from
-    ``lr`` to ``lr_final`` on a cosine over the steps. samples: list of
-    (x, graph, y[, n_mesh]) with y already scaled, or of ``device_sample``
-    dicts. Returns the per-epoch mean loss."""
+def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final=1e-5,
+          weights=(1.0, 1.0, 1.0)):
+    """Adam on the mean squared error over mesh nodes (per-component
+    ``weights`` on the three scaled targets, normalised to a mean of 1),
+    learning rate from ``lr`` to ``lr_final`` on a cosine over the steps.
+    samples: list of (x, graph, y[, n_mesh]) with y already scaled, or of
+    ``device_sample`` dicts. Returns the per-epoch mean loss."""
     torch = _torch()
     samples = [s if isinstance(s, dict) else device_sample(*s) for s in samples]
+    wq = np.asarray(weights, np.float64)
+    wq = torch.tensor(wq / wq.mean(), dtype=torch.float32, device="cuda")
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     per_epoch = -(-len(samples) // batch)
     total = max(1, epochs * per_epoch)
@@ -600,7 +604,7 @@ def train(net, samples, epochs=50, batch=16, lr=1e-3, seed=0, log=None, lr_final
                 grp["lr"] = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / total))
             x, g, y, m = concat_device([samples[i] for i in order[b0:b0 + batch]])
             opt.zero_grad(set_to_none=True)
-            loss = torch.sum(m * (net(x, g) - y) ** 2) / (3.0 * torch.sum(m))
+            loss = torch.sum(m * wq * (net(x, g) - y) ** 2) / (3.0 * torch.sum(m))
             loss.backward()
             opt.step()
             losses.append(loss.detach())

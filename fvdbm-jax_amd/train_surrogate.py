@@ -15,8 +15,10 @@
    as the targets and keeps only the network's inputs and targets, on the
    GPU; the meshes are dropped;
 4. trains MeshNet (Adam from ``--lr`` to ``--lr-final`` on a cosine,
-   ``--batch`` meshes per step, ``--epochs``) on all but the last ``--val``
-   cases and reports the relative L2 error of the prediction on those;
+   ``--batch`` meshes per step, ``--epochs``; the squared errors of the
+   three targets weighted by ``--loss-weights``) on all but the last
+   ``--val`` cases and reports the relative L2 error of the prediction on
+   those;
 5. writes the float32 weights, the target scaling and the run's record.
 """
 from __future__ import annotations
@@ -91,6 +93,132 @@ def steady_states(cases, tol=1e-6, check_every=500, max_steps=400_000, group_cel
     return res
 
 
+class _InProcess:
+    """The part of a process pool's interface ``_solve_groups`` uses, meshing
+    each group when it is asked for."""
+
+    class _Result:
+        def __init__(self, fn, items):
+            self.fn, self.items = fn, items
+
+        def get(self):
+            return [self.fn(i) for i in self.items]
+
+        def wait(self):
+            pass
+
+    def map_async(self, fn, items, chunksize=1):
+        return self._Result(fn, items)
+
+    def close(self):
+        pass
+
+
+def _solve_groups(seeds, dist, consume, group=100, tol=1e-6, check_every=1000, max_steps=600_000,
+                  workers=16, init_weights=None, data_seconds=None, log=print):
+    """Mesh the cases of ``seeds`` (distribution ``dist``) in a process pool,
+    ``group`` at a time, one group ahead of the solver; solve each group to
+    a residual below ``tol`` (warm-started from ``init_weights`` when given,
+    cold fallback per case) and hand ``consume(cases, fields, converged
+    indices)`` each group, then drop its meshes. The pool is forked here:
+    call before the process touches the GPU; ``workers`` 0 meshes in this
+    process instead (for profilers whose handlers a forked child inherits).
+    Stops after the group that passes ``data_seconds``. Returns timing and
+    count stats."""
+    global _DIST
+    _DIST = dist
+    seeds = list(seeds)
+    chunks = [seeds[i:i + group] for i in range(0, len(seeds), group)]
+    t0 = time.perf_counter()
+    if workers > 0:
+        pool = mp.get_context("fork").Pool(workers)  # forked before any GPU use
+    else:
+        pool = _InProcess()
+    pending = pool.map_async(_build, chunks[0], chunksize=2)
+    init_net = surrogate.MeshNet.load(init_weights) if init_weights else None
+    attempted = 0
+    t_solve = t_mesh_wait = 0.0
+    steps = []
+    for gi in range(len(chunks)):
+        tw = time.perf_counter()
+        cases = pending.get()
+        t_mesh_wait += time.perf_counter() - tw
+        if gi + 1 < len(chunks):
+            pending = pool.map_async(_build, chunks[gi + 1], chunksize=2)
+        ts = time.perf_counter()
+        fields = steady_states(cases, tol, check_every, max_steps, group_cells=1 << 62, log=None,
+                               net=init_net)
+        t_solve += time.perf_counter() - ts
+        attempted += len(cases)
+        ok = [i for i, f in enumerate(fields) if f["status"] == warmstart.CONVERGED]
+        steps += [fields[i]["steps"] for i in ok]
+        consume(cases, fields, ok)
+        log(f"group {gi}: {len(cases)} cases ({sum(c.n_cells for c in cases)} cells), {len(ok)} converged, "
+            f"median steps {np.median([f['steps'] for f in fields]):.0f}, "
+            f"{time.perf_counter() - t0:.1f} s elapsed")
+        del cases, fields
+        if data_seconds and time.perf_counter() - t0 > data_seconds and gi + 1 < len(chunks):
+            log(f"data time limit reached after {gi + 1} of {len(chunks)} groups")
+            pending.wait()
+            break
+    pool.close()
+    return dict(attempted=attempted, solve_s=t_solve, mesh_wait_s=t_mesh_wait, elapsed_s=time.perf_counter() - t0,
+                steps_median=float(np.median(steps)) if steps else None)
+
+
+def sample_builder(grids, resolution=True, workers=16):
+    """A ``consume`` for ``_solve_groups`` that keeps, per virtual-grid
+    size in ``grids``, each converged case's network inputs and unscaled
+    targets on the GPU; returns (consume, samples {grid: list}, info list)."""
+    samples = {g: [] for g in grids}
+    info = []
+    threads = ThreadPoolExecutor(max_workers=max(1, workers))
+
+    def consume(cases, fields, ok):
+        for grid in grids:
+            ins = list(threads.map(lambda i: surrogate.case_inputs(cases[i], grid, resolution), ok))
+            for i, (x, g, nm) in zip(ok, ins):
+                c, f = cases[i], fields[i]
+                y = surrogate.targets(f["rho"], f["vel"], c.u_in, (1.0, 1.0, 1.0), g["n"] - nm)
+                samples[grid].append(surrogate.device_sample(x, g, y, nm))
+        for i in ok:
+            info.append(dict(seed=cases[i].seed, cells=cases[i].n_cells, steps=fields[i]["steps"]))
+
+    return consume, samples, info
+
+
+def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.0, 1.0, 1.0), log=print):
+    """Scale the targets by their training-set standard deviation, train a
+    MeshNet on all but the last ``val`` samples, evaluate on those. Returns
+    (net, record dict)."""
+    torch = surrogate._torch()
+    tr, va = samples[:-val], samples[-val:]
+    with torch.no_grad():
+        ys = torch.cat([s["y"][s["mask"][:, 0] > 0] for s in tr]).double()
+        scale = ys.std(dim=0).cpu().numpy()
+        sc = torch.tensor(scale, dtype=torch.float32, device=ys.device)
+        for s in samples:
+            s["y"] /= sc
+    del ys
+    log(f"target scale {scale}")
+    net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid, resolution=resolution)
+    t2 = time.perf_counter()
+    hist = surrogate.train(net, tr, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, log=log,
+                           weights=weights)
+    t_train = time.perf_counter() - t2
+    err = surrogate.evaluate(net, va)
+    with torch.no_grad():
+        for s in samples:  # back to unscaled targets, so the samples can train another net
+            s["y"] *= sc
+    log(f"trained in {t_train:.1f} s; validation relative L2 (rho-1, ux, uy): "
+        f"median {np.median(err, axis=0)}, max {err.max(axis=0)}")
+    record = dict(train=len(tr), val=len(va), epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
+                  loss=hist, grid=list(grid), resolution=resolution, loss_weights=list(weights),
+                  val_rel_l2_median=np.median(err, axis=0).tolist(), val_rel_l2_max=err.max(axis=0).tolist(),
+                  train_s=t_train)
+    return net, record
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--cases", type=int, default=2000)
@@ -107,7 +235,10 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--loss-weights", default="1,1,1",
+                    help="weights of the (rho - 1, ux, uy) terms of the loss")
+    ap.add_argument("--workers", type=int, default=16,
+                    help="meshing processes (0: mesh in this process, for profiling)")
     ap.add_argument("--grid", default="32x16", help="virtual grid GXxGY (GY = 1: bands of x)")
     ap.add_argument("--no-resolution", action="store_true",
                     help="leave out the mesh-resolution input column")
@@ -119,78 +250,22 @@ def main(argv=None):
     def log(msg):
         print(msg, flush=True)
 
-    global _DIST
-    _DIST = porous.DISTRIBUTIONS[a.dist]
     grid = tuple(int(v) for v in a.grid.split("x"))
     resolution = not a.no_resolution
-    seeds = list(range(a.seed0, a.seed0 + a.cases))
-    chunks = [seeds[i:i + a.group] for i in range(0, len(seeds), a.group)]
-    t0 = time.perf_counter()
-    # the workers are forked here, before this process touches the GPU
-    pool = mp.get_context("fork").Pool(a.workers)
-    pending = pool.map_async(_build, chunks[0], chunksize=2)
-    init_net = surrogate.MeshNet.load(a.init_weights) if a.init_weights else None
-    samples, info = [], []
-    attempted = 0
-    t_solve = t_mesh_wait = 0.0
-    threads = ThreadPoolExecutor(max_workers=a.workers)
-    for gi in range(len(chunks)):
-        tw = time.perf_counter()
-        cases = pending.get()
-        t_mesh_wait += time.perf_counter() - tw
-        if gi + 1 < len(chunks):
-            pending = pool.map_async(_build, chunks[gi + 1], chunksize=2)
-        ts = time.perf_counter()
-        fields = steady_states(cases, a.tol, a.check_every, a.max_steps, group_cells=1 << 62,
-                               log=None, net=init_net)
-        t_solve += time.perf_counter() - ts
-        attempted += len(cases)
-        ok = [i for i, f in enumerate(fields) if f["status"] == warmstart.CONVERGED]
-        ins = list(threads.map(lambda i: surrogate.case_inputs(cases[i], grid, resolution), ok))
-        for i, (x, g, nm) in zip(ok, ins):
-            c, f = cases[i], fields[i]
-            y = surrogate.targets(f["rho"], f["vel"], c.u_in, (1.0, 1.0, 1.0), g["n"] - nm)
-            samples.append(surrogate.device_sample(x, g, y, nm))
-            info.append(dict(seed=c.seed, cells=c.n_cells, steps=f["steps"]))
-        log(f"group {gi}: {len(cases)} cases ({sum(c.n_cells for c in cases)} cells), {len(ok)} converged, "
-            f"median steps {np.median([f['steps'] for f in fields]):.0f}, "
-            f"{time.perf_counter() - t0:.1f} s elapsed")
-        del cases, fields, ins
-        if a.data_seconds and time.perf_counter() - t0 > a.data_seconds and gi + 1 < len(chunks):
-            log(f"data time limit reached after {gi + 1} of {len(chunks)} groups")
-            pending.wait()
-            break
-    pool.close()
-    threads.shutdown()
-    log(f"data: {len(samples)} steady states in {time.perf_counter() - t0:.1f} s "
-        f"(solves {t_solve:.1f} s, waiting for meshes {t_mesh_wait:.1f} s)")
-    torch = surrogate._torch()
-    tr, va = samples[:-a.val], samples[-a.val:]
-    # per-component scale of the targets over the training nodes
-    with torch.no_grad():
-        ys = torch.cat([s["y"][s["mask"][:, 0] > 0] for s in tr]).double()
-        scale = ys.std(dim=0).cpu().numpy()
-        sc = torch.tensor(scale, dtype=torch.float32, device=ys.device)
-        for s in samples:
-            s["y"] /= sc
-    del ys
-    log(f"target scale {scale}")
-    net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid, resolution=resolution)
-    t2 = time.perf_counter()
-    hist = surrogate.train(net, tr, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final, log=log)
-    t_train = time.perf_counter() - t2
-    err = surrogate.evaluate(net, va)
-    log(f"trained in {t_train:.1f} s; validation relative L2 (rho-1, ux, uy): "
-        f"median {np.median(err, axis=0)}, max {err.max(axis=0)}")
+    consume, samples, info = sample_builder([grid], resolution, a.workers)
+    stats = _solve_groups(range(a.seed0, a.seed0 + a.cases), porous.DISTRIBUTIONS[a.dist], consume,
+                          group=a.group, tol=a.tol, check_every=a.check_every, max_steps=a.max_steps,
+                          workers=a.workers, init_weights=a.init_weights, data_seconds=a.data_seconds,
+                          log=log)
+    log(f"data: {len(info)} steady states in {stats['elapsed_s']:.1f} s (solves {stats['solve_s']:.1f} s, "
+        f"waiting for meshes {stats['mesh_wait_s']:.1f} s)")
+    weights = tuple(float(v) for v in a.loss_weights.split(","))
+    net, rec = fit(samples[grid], a.val, grid, resolution, a.epochs, a.batch, a.lr, a.lr_final, weights, log)
     state = net.state()
-    record = dict(cases=attempted, dist=a.dist, seed0=a.seed0, converged=len(samples), train=len(tr),
-                  val=len(va), tol=a.tol, epochs=a.epochs, batch=a.batch, lr=a.lr, lr_final=a.lr_final,
-                  loss=hist, grid=list(grid), resolution=resolution,
+    record = dict(cases=stats["attempted"], dist=a.dist, seed0=a.seed0, converged=len(info), tol=a.tol,
                   init_weights=os.path.basename(a.init_weights) if a.init_weights else None,
-                  cells_median=float(np.median([i["cells"] for i in info])),
-                  val_rel_l2_median=np.median(err, axis=0).tolist(), val_rel_l2_max=err.max(axis=0).tolist(),
-                  solve_s=t_solve, train_s=t_train,
-                  solve_steps_median=float(np.median([i["steps"] for i in info])))
+                  cells_median=float(np.median([i["cells"] for i in info])), solve_s=stats["solve_s"],
+                  solve_steps_median=stats["steps_median"], **rec)
     state["record"] = np.array(json.dumps(record))
     np.savez(a.out, **state)
     log(json.dumps(record))
