@@ -282,11 +282,12 @@ def _batch_fp64(sub, best, fp64_steps):
 
 
 def steady_state(cases, weights, tol=1e-6, check_every=1000, max_steps=800_000, ref_cases=16,
-                 ref_tol=1e-10):
+                 ref_tol=1e-10, raw=None):
     """Cold against warm-started batched solves of the same unseen cases,
     both in FP32 to a per-step residual below ``tol``; then an FP64 run of
     the first ``ref_cases`` cases, continued from the cold solution to a
-    residual below ``ref_tol``, judges both final fields."""
+    residual below ``ref_tol``, judges both final fields. A ``raw`` dict
+    receives the per-case arrays (for studies run in groups)."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     sg = importlib.import_module("fvdbm-jax_amd.surrogate")
@@ -337,6 +338,12 @@ def steady_state(cases, weights, tol=1e-6, check_every=1000, max_steps=800_000, 
                    rel_l2_rho_cold_median=float(np.median(e["cold_rho"])),
                    rel_l2_rho_warm_median=float(np.median(e["warm_rho"])))
     cells = np.array([c.n_cells for c in cases])
+    if raw is not None:
+        raw.update(cells=cells, cold_steps=cold["steps"].copy(), warm_steps=warm["steps"].copy(),
+                   ideal_steps=ideal["steps"].copy(), ok=ok, du=du, dr=dr,
+                   converged_cold=cold["status"] == ws.CONVERGED, converged_warm=warm["status"] == ws.CONVERGED,
+                   fallback=warm["fallback"].copy(), wall_cold=cold["wall_s"], wall_warm=warm_s,
+                   inference=t_pred, ref=ref)
     return dict(
         cases=len(cases), cells=int(cells.sum()), cells_per_case=[int(cells.min()), int(cells.max())],
         tol=tol, check_every=check_every,
@@ -353,6 +360,63 @@ def steady_state(cases, weights, tol=1e-6, check_every=1000, max_steps=800_000, 
         inference_s=round(t_pred, 3), wall_ratio=round(cold["wall_s"] / warm_s, 3),
         final_rel_l2_u_warm_vs_cold_median=float(np.median(du)), final_rel_l2_u_warm_vs_cold_max=float(du.max()),
         final_rel_l2_rho_warm_vs_cold_median=float(np.median(dr)),
+        final_rel_l2_target=1e-4,
+        fp64_reference=ref)
+
+
+def _build_steady(seed):
+    fv = importlib.import_module("fvdbm-jax_amd")
+    return fv.porous.make_case(seed, dist=fv.porous.STEADY)
+
+
+def steady_study(seeds, weights, group, pool, ref_cases=16):
+    """``steady_state`` over many cases in groups of ``group``: ``pool``
+    (forked before the GPU was touched) meshes the next group while the
+    current one is solved, and one line is logged per group. The FP64
+    reference covers the first group's first ``ref_cases`` cases. Returns
+    the same record as ``steady_state`` over all cases."""
+    seeds = list(seeds)
+    chunks = [seeds[i:i + group] for i in range(0, len(seeds), group)]
+    pending = pool.map_async(_build_steady, chunks[0], chunksize=2)
+    raws = []
+    t0 = time.time()
+    for gi in range(len(chunks)):
+        cases = pending.get()
+        if gi + 1 < len(chunks):
+            pending = pool.map_async(_build_steady, chunks[gi + 1], chunksize=2)
+        raw = {}
+        steady_state(cases, weights, ref_cases=ref_cases if gi == 0 else 0, raw=raw)
+        raws.append(raw)
+        ok = raw["ok"]
+        log(f"steady group {gi + 1}/{len(chunks)}: {len(cases)} cases, step ratio median "
+            f"{np.median(raw['cold_steps'][ok] / np.maximum(raw['warm_steps'][ok], 1)):.3f}, "
+            f"{time.time() - t0:.0f}s")
+        del cases
+    cat = {k: np.concatenate([r[k] for r in raws]) for k in
+           ("cells", "cold_steps", "warm_steps", "ideal_steps", "ok", "du", "dr", "converged_cold",
+            "converged_warm", "fallback")}
+    wall_cold = sum(r["wall_cold"] for r in raws)
+    wall_warm = sum(r["wall_warm"] for r in raws)
+    ok = cat["ok"]
+    ratio = cat["cold_steps"][ok] / np.maximum(cat["warm_steps"][ok], 1)
+    ref = raws[0]["ref"]
+    return dict(
+        cases=len(seeds), groups=len(chunks), group=group, cells=int(cat["cells"].sum()),
+        cells_per_case=[int(cat["cells"].min()), int(cat["cells"].max())], tol=1e-6, check_every=1000,
+        converged_cold=int(cat["converged_cold"].sum()), converged_warm=int(cat["converged_warm"].sum()),
+        fallbacks=int(cat["fallback"].sum()),
+        steps_cold_median=float(np.median(cat["cold_steps"])), steps_warm_median=float(np.median(cat["warm_steps"])),
+        steps_cold_total=int(cat["cold_steps"].sum()), steps_warm_total=int(cat["warm_steps"].sum()),
+        steps_exact_fields_median=float(np.median(cat["ideal_steps"])),
+        step_ratio_median=float(np.median(ratio)) if ratio.size else None,
+        step_ratio_p10_p90=[float(np.percentile(ratio, 10)), float(np.percentile(ratio, 90))] if ratio.size else None,
+        step_ratio_total=float(cat["cold_steps"].sum() / max(cat["warm_steps"].sum(), 1)),
+        step_ratio_target=3.0,
+        wall_cold_s=round(wall_cold, 3), wall_warm_s=round(wall_warm, 3),
+        inference_s=round(sum(r["inference"] for r in raws), 3), wall_ratio=round(wall_cold / wall_warm, 3),
+        final_rel_l2_u_warm_vs_cold_median=float(np.median(cat["du"])),
+        final_rel_l2_u_warm_vs_cold_max=float(cat["du"].max()),
+        final_rel_l2_rho_warm_vs_cold_median=float(np.median(cat["dr"])),
         final_rel_l2_target=1e-4,
         fp64_reference=ref)
 
@@ -392,6 +456,9 @@ def main():
     ap.add_argument("--steady-ref-cases", type=int, default=16,
                     help="of those, cases that get the FP64 reference solution")
     ap.add_argument("--steady-seed0", type=int, default=1_000_000)
+    ap.add_argument("--steady-group", type=int, default=0,
+                    help="solve the steady-state study in batches of this many cases, meshing the next "
+                         "batch meanwhile (for studies too large for one handle, e.g. 1024 cases)")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="porous cases of the batched workload for the batched throughput line (0: skip)")
@@ -417,13 +484,21 @@ def main():
         parts &= {"single"}
 
     steady_cases = None
-    if "steady" in parts and args.steady_cases > 0 and os.path.exists(args.weights):
+    steady_pool = None
+    steady_seeds = range(args.steady_seed0, args.steady_seed0 + args.steady_cases)
+    if ("steady" in parts and args.steady_cases > 0 and os.path.exists(args.weights)
+            and 0 < args.steady_group < args.steady_cases):
+        # groups meshed by a pool forked now, before anything touches the GPU
+        import multiprocessing as mp
+        ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
+        steady_pool = (mp.get_context("fork").Pool(args.mesh_workers) if args.mesh_workers > 0
+                       else ts._InProcess())
+    elif "steady" in parts and args.steady_cases > 0 and os.path.exists(args.weights):
         # meshed in a fork pool before anything touches the GPU
         fvp = importlib.import_module("fvdbm-jax_amd")
         ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
         t0 = time.time()
-        steady_cases = ts.build_cases(range(args.steady_seed0, args.steady_seed0 + args.steady_cases),
-                                      workers=max(1, args.mesh_workers), dist=fvp.porous.STEADY)
+        steady_cases = ts.build_cases(steady_seeds, workers=max(1, args.mesh_workers), dist=fvp.porous.STEADY)
         log(f"steady-state study: {len(steady_cases)} cases meshed in {time.time() - t0:.1f}s")
     batch_cases = None
     if "batched" in parts and args.batch_cases > 0:
@@ -529,9 +604,14 @@ def main():
         out["batched"]["workload"] = ("porous channels, 5-40 discs, r in [0.02,0.08], 20k-200k cells, "
                                       "u_in in [0.02,0.1], tau in [0.55,0.9], seeds 0..")
         log(f"batched {time.time() - t0:.1f}s: {out['batched']}")
-    if steady_cases is not None:
+    if steady_cases is not None or steady_pool is not None:
         t0 = time.time()
-        out["steady_state"] = steady_state(steady_cases, args.weights, ref_cases=args.steady_ref_cases)
+        if steady_pool is not None:
+            out["steady_state"] = steady_study(steady_seeds, args.weights, args.steady_group, steady_pool,
+                                               ref_cases=args.steady_ref_cases)
+            steady_pool.close()
+        else:
+            out["steady_state"] = steady_state(steady_cases, args.weights, ref_cases=args.steady_ref_cases)
         out["steady_state"]["workload"] = (
             "unseen porous channels (seeds from %d), 3-8 discs, r in [0.02,0.06], gap 0.03, "
             "20k-200k cells, u_in in [0.02,0.05], tau in [0.55,0.9]" % args.steady_seed0)

@@ -15,11 +15,13 @@
 //   aggregate: z = [h | sum_e w h_j | sum_e w ex h_j | sum_e w ey h_j]
 //          over each node's CSR edge list, and its transpose for dh.
 //
-// MFMA use: __builtin_amdgcn_mfma_f32_16x16x16bf16_1k. For one 64-lane wave
-// with i = lane % 16 and g = lane / 16:
+// MFMA use: fragments follow the 16x16x16 bf16 layout. For one 64-lane
+// wave with i = lane % 16 and g = lane / 16:
 //   A (16x16, MxK): lane holds A[i][4g .. 4g+3]
 //   B (16x16, KxN): lane holds B[4g .. 4g+3][i]
 //   D (16x16, MxN): lane holds D[4g + r][i], r = 0..3
+// K is consumed 32 at a time by gfx950's 16x16x32 bf16 MFMA (mfma2: two
+// such fragments per operand), 16 at a time only for a 16-wide input.
 // The dense layers are computed transposed (hidden units as MFMA rows, data
 // rows as MFMA columns): then the D fragment of one layer is, lane by lane,
 // exactly the B fragment the next layer needs, so the hidden activations
@@ -41,6 +43,8 @@ constexpr int CHUNK = 1024;  // rows per partial product in wgrad / colsum
 constexpr int RGROUP = 32;   // partials summed per thread in the first reduction pass
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 bits, round to nearest even (finite inputs)
@@ -52,6 +56,19 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Two consecutive K = 16 steps, c + A0 B0 + A1 B1, as one gfx950
+// 16x16x32 MFMA. Each lane gives that instruction eight k values of A
+// and the same eight k positions of B: here its four of (a0, b0) and its
+// four of (a1, b1). A and B use one assignment of the 32 k indices to
+// (lane group, slot), so the product sums exactly the 32 products of the
+// two 16-blocks, in the instruction's own order (FP32 accumulation).
+__device__ __forceinline__ f4 mfma2(s4 a0, s4 a1, s4 b0, s4 b1, f4 c) {
+  const s8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const s8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c,
+                                                  0, 0, 0);
 }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
@@ -98,14 +115,30 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const float *__restrict__ x, in
     f4 z[4];
 #pragma unroll
     for (int hb = 0; hb < 4; ++hb) z[hb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < K / 16; ++ks) {
+    // the B fragment of x's 16-column block ks: x[row][16 ks + 4g .. +3]
+    auto xfrag = [&](int ks) {
       const float4 xv = ok ? *reinterpret_cast<const float4 *>(x + (size_t)row * K + ks * 16 + 4 * g)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
-      const s4 b = bf4(xv);
+      return bf4(xv);
+    };
+    if constexpr (K % 32 == 0) {
 #pragma unroll
-      for (int hb = 0; hb < 4; ++hb)
-        z[hb] = mfma(lds4(sW1T + (hb * 16 + i) * KP + ks * 16 + 4 * g), b, z[hb]);
+      for (int kp = 0; kp < K / 32; ++kp) {
+        const s4 b0 = xfrag(2 * kp), b1 = xfrag(2 * kp + 1);
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb) {
+          const unsigned short *w = sW1T + (hb * 16 + i) * KP + kp * 32 + 4 * g;
+          z[hb] = mfma2(lds4(w), lds4(w + 16), b0, b1, z[hb]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < K / 16; ++ks) {
+        const s4 b = xfrag(ks);
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb)
+          z[hb] = mfma(lds4(sW1T + (hb * 16 + i) * KP + ks * 16 + 4 * g), b, z[hb]);
+      }
     }
     s4 hfrag[4];
 #pragma unroll
@@ -119,8 +152,10 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const float *__restrict__ x, in
     for (int ob = 0; ob < NO / 16; ++ob) {
       f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        acc = mfma(lds4(sW2T + (ob * 16 + i) * HP + ks * 16 + 4 * g), hfrag[ks], acc);
+      for (int kp = 0; kp < 2; ++kp) {
+        const unsigned short *w = sW2T + (ob * 16 + i) * HP + kp * 32 + 4 * g;
+        acc = mfma2(lds4(w), lds4(w + 16), hfrag[2 * kp], hfrag[2 * kp + 1], acc);
+      }
       if (ok) {
         const int o = ob * 16 + 4 * g;
         *reinterpret_cast<float4 *>(y + (size_t)row * NO + o) =
@@ -161,27 +196,56 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(const float *__restrict__ x, in
       z[hb] = f4{0.f, 0.f, 0.f, 0.f};
       dh[hb] = f4{0.f, 0.f, 0.f, 0.f};
     }
-#pragma unroll
-    for (int ks = 0; ks < K / 16; ++ks) {
+    // B fragments of x's and dy's 16-column block ks, and the A fragment
+    // W1^T[hidden block hb][k block ks] read down sW1's columns
+    auto xfrag = [&](int ks) {
       const float4 xv = ok ? *reinterpret_cast<const float4 *>(x + (size_t)row * K + ks * 16 + 4 * g)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
-      const s4 b = bf4(xv);
+      return bf4(xv);
+    };
+    auto dyfrag = [&](int os) {
+      const float4 dv = ok ? *reinterpret_cast<const float4 *>(dy + (size_t)row * NO + os * 16 + 4 * g)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      return bf4(dv);
+    };
+    auto w1t = [&](int ks, int hb) {
+      const unsigned short *p = sW1 + (ks * 16 + 4 * g) * HP + hb * 16 + i;
+      return s4{(short)p[0], (short)p[HP], (short)p[2 * HP], (short)p[3 * HP]};
+    };
+    if constexpr (K % 32 == 0) {
 #pragma unroll
-      for (int hb = 0; hb < 4; ++hb) {
-        const unsigned short *p = sW1 + (ks * 16 + 4 * g) * HP + hb * 16 + i;
-        const s4 a = s4{(short)p[0], (short)p[HP], (short)p[2 * HP], (short)p[3 * HP]};
-        z[hb] = mfma(a, b, z[hb]);
+      for (int kp = 0; kp < K / 32; ++kp) {
+        const s4 b0 = xfrag(2 * kp), b1 = xfrag(2 * kp + 1);
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb) z[hb] = mfma2(w1t(2 * kp, hb), w1t(2 * kp + 1, hb), b0, b1, z[hb]);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < K / 16; ++ks) {
+        const s4 b = xfrag(ks);
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb) z[hb] = mfma(w1t(ks, hb), b, z[hb]);
       }
     }
     // dh^T [hidden][rows] = W2 [hidden][out] . dy^T [out][rows]
+    if constexpr (NO % 32 == 0) {
 #pragma unroll
-    for (int os = 0; os < NO / 16; ++os) {
-      const float4 dv = ok ? *reinterpret_cast<const float4 *>(dy + (size_t)row * NO + os * 16 + 4 * g)
-                           : make_float4(0.f, 0.f, 0.f, 0.f);
-      const s4 b = bf4(dv);
+      for (int op = 0; op < NO / 32; ++op) {
+        const s4 b0 = dyfrag(2 * op), b1 = dyfrag(2 * op + 1);
 #pragma unroll
-      for (int hb = 0; hb < 4; ++hb)
-        dh[hb] = mfma(lds4(sW2 + (hb * 16 + i) * NP + os * 16 + 4 * g), b, dh[hb]);
+        for (int hb = 0; hb < 4; ++hb) {
+          const unsigned short *w = sW2 + (hb * 16 + i) * NP + op * 32 + 4 * g;
+          dh[hb] = mfma2(lds4(w), lds4(w + 16), b0, b1, dh[hb]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int os = 0; os < NO / 16; ++os) {
+        const s4 b = dyfrag(os);
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb)
+          dh[hb] = mfma(lds4(sW2 + (hb * 16 + i) * NP + os * 16 + 4 * g), b, dh[hb]);
+      }
     }
     s4 dzfrag[4];
 #pragma unroll
@@ -206,8 +270,10 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(const float *__restrict__ x, in
       for (int kb = 0; kb < K / 16; ++kb) {
         f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-          acc = mfma(lds4(sW1 + (kb * 16 + i) * HP + ks * 16 + 4 * g), dzfrag[ks], acc);
+        for (int kp = 0; kp < 2; ++kp) {
+          const unsigned short *w = sW1 + (kb * 16 + i) * HP + kp * 32 + 4 * g;
+          acc = mfma2(lds4(w), lds4(w + 16), dzfrag[2 * kp], dzfrag[2 * kp + 1], acc);
+        }
         if (ok)
           *reinterpret_cast<float4 *>(dx + (size_t)row * K + kb * 16 + 4 * g) =
               make_float4(acc[0], acc[1], acc[2], acc[3]);
@@ -268,17 +334,18 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
     }
     __syncthreads();
 #pragma unroll
-    for (int ms = 0; ms < R / 16; ++ms) {
-      const int m = ms * 16 + 4 * g;  // this lane's four rows of the 16-row slab
+    for (int mp = 0; mp < R / 32; ++mp) {
+      const int m = mp * 32 + 4 * g;  // this lane's four rows of each 16-row slab of the pair
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
         const int blk = wave + 4 * p;
         if (blk < NBLK) {
           const int kb = blk / (NB / 16), nb = blk % (NB / 16);
-          // A fragment: feature kb*16+i, rows m..m+3; B fragment: feature nb*16+i, same rows
-          const s4 a = *reinterpret_cast<const s4 *>(sA + (kb * 16 + i) * RP + m);
-          const s4 b = *reinterpret_cast<const s4 *>(sB + (nb * 16 + i) * RP + m);
-          acc[p] = mfma(a, b, acc[p]);
+          // A fragments: feature kb*16+i, rows m..m+3 and m+16..m+19; B: feature nb*16+i, same rows
+          const unsigned short *pa = sA + (kb * 16 + i) * RP + m;
+          const unsigned short *pb = sB + (nb * 16 + i) * RP + m;
+          acc[p] = mfma2(*reinterpret_cast<const s4 *>(pa), *reinterpret_cast<const s4 *>(pa + 16),
+                         *reinterpret_cast<const s4 *>(pb), *reinterpret_cast<const s4 *>(pb + 16), acc[p]);
         }
       }
     }
