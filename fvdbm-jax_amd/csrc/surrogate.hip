@@ -290,6 +290,37 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(const float *__restrict__ x, in
 // read. The output 16x16 blocks are dealt round-robin to the four waves;
 // each wave's accumulators live in registers across the chunk.
 // ---------------------------------------------------------------------------
+// Stage rows [base, base + R) of X [rows][K] (rows from r1 on read as 0)
+// into s as bf16, feature-major ([K][RS]). Work item e takes the 4x4 block
+// of features 4c .. 4c+3 and rows 4q .. 4q+3 (q = e % (R/4) fastest, so
+// neighbouring lanes write neighbouring 8-byte words of one feature row):
+// four 16-byte loads, one per row, and four 8-byte LDS stores, one per
+// feature. X rows must be 16-byte aligned (K a multiple of 4).
+template <int K, int R, int RS>
+__device__ __forceinline__ void stage_transposed(const float *__restrict__ X, int base, int r1,
+                                                 unsigned short *__restrict__ s) {
+  static_assert(K % 4 == 0 && R % 4 == 0, "4x4 blocks");
+  for (int e = threadIdx.x; e < (K / 4) * (R / 4); e += blockDim.x) {
+    const int q = e % (R / 4), c = e / (R / 4);
+    float4 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = base + 4 * q + j;
+      v[j] = row < r1 ? *reinterpret_cast<const float4 *>(X + (size_t)row * K + 4 * c)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const float col[4][4] = {{v[0].x, v[1].x, v[2].x, v[3].x},
+                             {v[0].y, v[1].y, v[2].y, v[3].y},
+                             {v[0].z, v[1].z, v[2].z, v[3].z},
+                             {v[0].w, v[1].w, v[2].w, v[3].w}};
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      *reinterpret_cast<uint2 *>(s + (4 * c + f) * RS + 4 * q) =
+          make_uint2((uint32_t)f2bf(col[f][0]) | ((uint32_t)f2bf(col[f][1]) << 16),
+                     (uint32_t)f2bf(col[f][2]) | ((uint32_t)f2bf(col[f][3]) << 16));
+  }
+}
+
 template <int KA, int NB>
 __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
                                                const float *__restrict__ B, int M,
@@ -308,30 +339,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A,
   for (int p = 0; p < PER; ++p) acc[p] = f4{0.f, 0.f, 0.f, 0.f};
   for (int base = r0; base < r1; base += R) {
     __syncthreads();
-    // thread e stages feature c = e % K of rows 4q .. 4q+3 (q = e / K):
-    // four coalesced loads, one 8-byte LDS store
-    for (int e = threadIdx.x; e < KA * (R / 4); e += blockDim.x) {
-      const int c = e % KA, q = e / KA;
-      unsigned short v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = base + 4 * q + j;
-        v[j] = row < r1 ? f2bf(A[(size_t)row * KA + c]) : (unsigned short)0;
-      }
-      *reinterpret_cast<uint2 *>(sA + c * RP + 4 * q) =
-          make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
-    }
-    for (int e = threadIdx.x; e < NB * (R / 4); e += blockDim.x) {
-      const int c = e % NB, q = e / NB;
-      unsigned short v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = base + 4 * q + j;
-        v[j] = row < r1 ? f2bf(B[(size_t)row * NB + c]) : (unsigned short)0;
-      }
-      *reinterpret_cast<uint2 *>(sB + c * RP + 4 * q) =
-          make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
-    }
+    stage_transposed<KA, R, RP>(A, base, r1, sA);
+    stage_transposed<NB, R, RP>(B, base, r1, sB);
     __syncthreads();
 #pragma unroll
     for (int mp = 0; mp < R / 32; ++mp) {
@@ -556,6 +565,10 @@ const char *fvdbm_nn_last_error(void) { return g_err.c_str(); }
 
 int fvdbm_nn_chunk_rows(void) { return CHUNK; }
 
+// the dense kernels move four floats per lane: their row arrays must be
+// 16-byte aligned
+static bool aligned16(const void *p) { return p == nullptr || ((uintptr_t)p & 15u) == 0; }
+
 #define NN_DISPATCH_K_NO(K_, NO_, CALL) \
   if (k == K_ && n_out == NO_) {        \
     CALL(K_, NO_);                      \
@@ -567,6 +580,7 @@ int fvdbm_nn_mlp_forward(const float *x, int64_t m, int32_t k, const float *w1, 
   if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
   if (m == 0) return FVDBM_OK;
   if (!x || !w1 || !b1 || !w2 || !b2 || !y) return nn_fail("null pointer");
+  if (!aligned16(x) || !aligned16(y)) return nn_fail("x and y must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int mi = (int)m;
 #define CALL_FWD(K_, NO_) launch_fwd<K_, NO_>(x, mi, w1, b1, w2, b2, y, s)
@@ -584,6 +598,8 @@ int fvdbm_nn_mlp_backward(const float *x, int64_t m, int32_t k, const float *w1,
   if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
   if (m == 0) return FVDBM_OK;
   if (!x || !w1 || !b1 || !w2 || !dy || !dz || !h) return nn_fail("null pointer");
+  if (!aligned16(x) || !aligned16(dy) || !aligned16(dx) || !aligned16(dz) || !aligned16(h))
+    return nn_fail("x, dy, dx, dz and h must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int mi = (int)m;
 #define CALL_BWD(K_, NO_) launch_bwd<K_, NO_>(x, mi, w1, b1, w2, dy, dx, dz, h, s)
@@ -599,6 +615,7 @@ int fvdbm_nn_wgrad(const float *a, const float *b, int64_t m, int32_t ka, int32_
                    float *out, void *stream) {
   if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
   if (!a || !b || !partial || !out) return nn_fail("null pointer");
+  if (!aligned16(a) || !aligned16(b)) return nn_fail("a and b must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int mi = (int)m;
   const int nch = (mi + CHUNK - 1) / CHUNK;
@@ -619,6 +636,7 @@ int fvdbm_nn_wgrad_noalloc(const float *a, const float *b, int64_t m, int32_t ka
                            float *out, void *stream) {
   if (m < 0 || m > INT32_MAX) return nn_fail("row count out of range");
   if (!a || !b || !out) return nn_fail("null pointer");
+  if (!aligned16(a) || !aligned16(b)) return nn_fail("a and b must be 16-byte aligned");
   if (!((ka == 16 && nb == 64) || (ka == 64 && nb == 64) || (ka == 256 && nb == 64) ||
         (ka == 64 && nb == 16)))
     return nn_fail("unsupported wgrad widths");

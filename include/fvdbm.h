@@ -258,7 +258,9 @@ int fvdbm_layout_order(const fvdbm_mesh_desc *mesh, int32_t precision, int32_t r
  * no learned component). Device pointers to row-major float32 arrays; the
  * stream is a hipStream_t (NULL = the default stream). Dense layers use
  * bf16 MFMA operands with float32 accumulation; weights are float32 master
- * copies rounded to bf16 on load. Hidden width 64.
+ * copies rounded to bf16 on load. Hidden width 64. The row arrays of the
+ * dense kernels (x, y, dy, dx, dz, h; a and b of the weight gradient) must
+ * be 16-byte aligned: a misaligned one fails with FVDBM_ERR_INVALID.
  * ------------------------------------------------------------------------ */
 const char *fvdbm_nn_last_error(void);
 /* Rows per partial product of fvdbm_nn_wgrad / fvdbm_nn_colsum: their
