@@ -107,3 +107,19 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_dense_kernels_reject_misaligned_rows():
+    """The dense surrogate kernels load four floats per lane: each entry
+    point rejects a row array that is not 16-byte aligned before it launches
+    anything (checked on the host, so no GPU is needed)."""
+    L = _lib.lib()
+    ok, bad = 0x10000, 0x10004  # never dereferenced: the check comes first
+    rc = L.fvdbm_nn_mlp_forward(bad, 32, 64, ok, ok, ok, ok, 64, ok, None)
+    assert rc != 0 and b"16-byte aligned" in L.fvdbm_nn_last_error()
+    rc = L.fvdbm_nn_mlp_backward(ok, 32, 64, ok, ok, ok, 64, bad, None, ok, ok, None)
+    assert rc != 0 and b"16-byte aligned" in L.fvdbm_nn_last_error()
+    rc = L.fvdbm_nn_wgrad(ok, bad, 32, 64, 64, ok, ok, None)
+    assert rc != 0 and b"16-byte aligned" in L.fvdbm_nn_last_error()
+    rc = L.fvdbm_nn_wgrad_noalloc(bad, ok, 32, 64, 64, ok, None)
+    assert rc != 0 and b"16-byte aligned" in L.fvdbm_nn_last_error()
