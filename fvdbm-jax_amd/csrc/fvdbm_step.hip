@@ -237,8 +237,7 @@ __device__ __forceinline__ void Buf::st<double>(double v, uint32_t voff, uint32_
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, voff, soff, 0);
 }
 
-// LDS cell records written in 16-byte pieces: a cell's 9 pdfs padded to 48
-// bytes (FP32) or 80 bytes (FP64), so a neighbour is 3 (5) aligned reads.
+// LDS cell records: a cell's 9 pdfs, read and written one value at a time.
 template <typename T>
 struct LdsCell;
 // Records of Q = 9 values with an odd stride in 32-bit words (9 for FP32;
