@@ -189,17 +189,17 @@ def sample_builder(grids, resolution=True, workers=16):
 
 def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.0, 1.0, 1.0), log=print):
     """Scale the targets by their training-set standard deviation, train a
-    MeshNet on all but the last ``val`` samples, evaluate on those. Returns
-    (net, record dict)."""
+    MeshNet on all but the last ``val`` samples, evaluate on those. The
+    samples keep their unscaled targets (the scaled ones are copies), so
+    they can train another net. Returns (net, record dict)."""
     torch = surrogate._torch()
-    tr, va = samples[:-val], samples[-val:]
     with torch.no_grad():
-        ys = torch.cat([s["y"][s["mask"][:, 0] > 0] for s in tr]).double()
+        ys = torch.cat([s["y"][s["mask"][:, 0] > 0] for s in samples[:-val]]).double()
         scale = ys.std(dim=0).cpu().numpy()
         sc = torch.tensor(scale, dtype=torch.float32, device=ys.device)
-        for s in samples:
-            s["y"] /= sc
+        scaled = [dict(s, y=s["y"] / sc) for s in samples]
     del ys
+    tr, va = scaled[:-val], scaled[-val:]
     log(f"target scale {scale}")
     net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid, resolution=resolution)
     t2 = time.perf_counter()
@@ -207,12 +207,11 @@ def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.
                            weights=weights)
     t_train = time.perf_counter() - t2
     err = surrogate.evaluate(net, va)
-    with torch.no_grad():
-        for s in samples:  # back to unscaled targets, so the samples can train another net
-            s["y"] *= sc
+    ntr, nva = len(tr), len(va)
+    del scaled, tr, va
     log(f"trained in {t_train:.1f} s; validation relative L2 (rho-1, ux, uy): "
         f"median {np.median(err, axis=0)}, max {err.max(axis=0)}")
-    record = dict(train=len(tr), val=len(va), epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
+    record = dict(train=ntr, val=nva, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
                   loss=hist, grid=list(grid), resolution=resolution, loss_weights=list(weights),
                   val_rel_l2_median=np.median(err, axis=0).tolist(), val_rel_l2_max=err.max(axis=0).tolist(),
                   train_s=t_train)

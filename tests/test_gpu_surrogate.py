@@ -153,3 +153,34 @@ def test_training_reduces_loss(gpu):
     net = sg.MeshNet(sg.init_params(seed=0, layers=2), layers=2)
     hist = sg.train(net, samples, epochs=30, batch=2, lr=3e-3)
     assert hist[-1] < 0.2 * hist[0], hist
+
+
+def test_loss_weights_train_only_the_weighted_targets(gpu):
+    """train_surrogate.fit with loss weights (1, 0, 0): the first target's
+    error falls far more than the two unweighted ones', and fit leaves the
+    samples' unscaled targets untouched (it trains on scaled copies)."""
+    torch = _torch()
+    ts = importlib.import_module("fvdbm-jax_amd.train_surrogate")
+    samples = []
+    for s in range(4):
+        c, x, g = _small_graph(s)
+        p = c.mesher.points
+        y = np.stack([np.sin(3 * p[:, 0]), p[:, 1] * 2, np.cos(2 * p[:, 0] + p[:, 1])], 1)
+        y = np.concatenate([y, np.zeros((g["n"] - len(p), 3))]).astype(np.float32)
+        samples.append(sg.device_sample(x, g, y, len(p)))
+    before = [s["y"].clone() for s in samples]
+    net0 = sg.MeshNet(seed=0)
+    sc = torch.tensor([float(s) for s in np.std(np.concatenate(
+        [b[s["mask"][:, 0] > 0].cpu().numpy() for b, s in zip(before[:-1], samples[:-1])]), axis=0)],
+        device="cuda")
+    e0 = sg.evaluate(net0, [dict(s, y=s["y"] / sc) for s in samples[:-1]]).mean(axis=0)
+    net, rec = ts.fit(samples, 1, (32, 16), False, epochs=80, batch=3, lr=3e-3, lr_final=1e-3,
+                      weights=(1.0, 0.0, 0.0), log=lambda m: None)
+    for b, s in zip(before, samples):
+        assert torch.equal(b, s["y"])
+    e1 = sg.evaluate(net, [dict(s, y=s["y"] / sc) for s in samples[:-1]]).mean(axis=0)
+    gain = e1 / e0
+    print("relative L2 before", e0, "after", e1, flush=True)
+    assert gain[0] < 0.5, (e0, e1)
+    assert gain[0] < 0.7 * min(gain[1], gain[2]), (e0, e1)
+    assert rec["loss_weights"] == [1.0, 0.0, 0.0]
