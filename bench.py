@@ -249,11 +249,22 @@ def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_ca
                 masked=masked, fp32_vs_fp64=fp64)
 
 
-def _masked_summary(st, masked_steps, check_every):
+def _masked_summary(st, masked_steps, check_every, chunk=10_000):
+    """Counts per outcome of a masked run; for the cases still running at the
+    cap, how many still gain mass (mean density up over the last ``chunk``
+    steps) and their median density, its growth and residual."""
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     conv = st["status"] == ws.CONVERGED
     running = st["active"]
-    return dict(max_steps=masked_steps, check_every=check_every, runaway_mean_rho=ws.RUNAWAY_RHO,
+    trend = None
+    if running.any():
+        grow = (st["rho_last"] - st["rho_prev"])[running]
+        trend = dict(gaining_mass=int((grow > 0).sum()),
+                     mean_rho_median=float(np.median(st["rho_last"][running])),
+                     mean_rho_growth_per_10k_steps_median=float(np.median(grow) * 10_000 / chunk),
+                     residual_median=float(np.median(st["residual"][running])))
+    return dict(running_trend=trend, max_steps=masked_steps, check_every=check_every,
+                runaway_mean_rho=ws.RUNAWAY_RHO,
                 wall_s=round(st["wall_s"], 2), converged=int(conv.sum()),
                 runaway=int(st["runaway"].sum()), running=int(running.sum()),
                 diverged=int((st["status"] == ws.DIVERGED).sum()),

@@ -101,16 +101,20 @@ def run_until(solver, max_steps, check_every=2000, tol=1e-6, rho_max=RUNAWAY_RHO
     has no steady state, its density grows without bound while the velocity
     residual levels off, so the residual alone never stops it. The density
     is read every ``chunk`` steps. Returns the status dict of
-    ``solver.status()`` plus ``runaway`` (bool per case) and ``wall_s``."""
+    ``solver.status()`` plus ``runaway`` (bool per case), ``wall_s``, and
+    ``rho_last``/``rho_prev`` (mean density per case at the last two reads,
+    ``chunk`` steps apart: the trend of a case still running)."""
     t0 = time.perf_counter()
     runaway = np.zeros(solver.n, bool)
     done = 0
     st = solver.status()
+    rho_prev = rho_last = mean_density(solver)
     while done < max_steps and st["active"].any():
         n = min(chunk, max_steps - done)
         st = solver.run(n, check_every, tol, 0.0)
         done += n
-        high = (mean_density(solver) > rho_max) & st["active"]
+        rho_prev, rho_last = rho_last, mean_density(solver)
+        high = (rho_last > rho_max) & st["active"]
         if high.any():
             runaway |= high
             solver.set_active((st["active"] & ~high).astype(np.int32))
@@ -121,6 +125,7 @@ def run_until(solver, max_steps, check_every=2000, tol=1e-6, rho_max=RUNAWAY_RHO
     out = dict(st)
     out["runaway"] = runaway
     out["wall_s"] = time.perf_counter() - t0
+    out["rho_last"], out["rho_prev"] = rho_last, rho_prev
     return out
 
 
