@@ -27,7 +27,7 @@ workload (5-40 discs, 20k-200k cells, seeds 0..) in one handle: the
 autotuner picks tile size and kernel variant on the first 32 cases (every
 candidate checked bitwise), then MLUPS of the batched step over all their
 cells, then a masked run to a 1e-6 residual capped at ``--masked-steps``
-(200k) steps, with a runaway stop (mean density past 2: most of these
+(700k: every case of the 256 is decided by 630k) steps, with a runaway stop (mean density past 2: most of these
 channels are choked and have no steady state), reported as converged /
 runaway / still-running counts; FP32 against FP64 on 16 cases after 20k
 steps.
@@ -196,7 +196,7 @@ def fp64_cavity(fv):
     return fp64_single(fv, c, f, n, 1000)
 
 
-def batched(triples, steps=1000, masked_steps=200_000, check_every=2000, tune_cases=32,
+def batched(triples, steps=1000, masked_steps=700_000, check_every=2000, tune_cases=32,
             fp64_cases=16, fp64_steps=20_000, block=None):
     """Throughput of one batched launch over uneven porous cases (the
     batched workload's distribution), then a masked run: every case steps
@@ -474,7 +474,7 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="porous cases of the batched workload for the batched throughput line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=1000, help="timed steps of the batched launch")
-    ap.add_argument("--masked-steps", type=int, default=200_000,
+    ap.add_argument("--masked-steps", type=int, default=700_000,
                     help="step cap of the batched masked run (0: skip it)")
     ap.add_argument("--mesh-workers", type=int, default=16,
                     help="processes meshing the porous cases (0: in this process, e.g. under a "
