@@ -173,3 +173,28 @@ def test_nonfinite_case_stops_as_diverged(gpu):
     st = b.run(max_steps=400, check_every=100, tol=1e-12)
     assert st["status"].tolist() == [0, 2] and st["active"].tolist() == [True, False]
     assert np.isnan(st["residual"][1])
+
+
+def test_fp64_batch_equals_single_solves_and_oracle(gpu):
+    """The batched handle in FP64 (the precision the steady-state reference
+    runs in): each case after an odd number of steps is bitwise equal to its
+    FP64 single solve, and the first case to the FP64 oracle."""
+    from cases import oracle_env, oracle_state
+    from oracle import oracle
+    cs = _cases(3)
+    b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs], precision="fp64")
+    b.step(61)
+    pdf = b.get_pdf()
+    assert pdf.dtype == np.float64
+    for k, (m, dyn, c, f, n) in enumerate(cs):
+        env = fv.Environment(c, f, n, precision="fp64")
+        env.init()
+        env.step(61)
+        assert np.array_equal(b.case_pdf(k, pdf), env.cells.pdf), k
+        env.close()
+    m, dyn, c, f, n = cs[0]
+    oenv = oracle_env(c, f, n, "f64")
+    st = oracle_state(oenv, n, "f64")
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 61, "f64")
+    assert np.array_equal(b.case_pdf(0, pdf), st["cell_pdf"])
+    b.close()
