@@ -187,7 +187,8 @@ def sample_builder(grids, resolution=True, workers=16):
     return consume, samples, info
 
 
-def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.0, 1.0, 1.0), log=print):
+def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.0, 1.0, 1.0), log=print,
+        layers=surrogate.LAYERS):
     """Scale the targets by their training-set standard deviation, train a
     MeshNet on all but the last ``val`` samples, evaluate on those. The
     samples keep their unscaled targets (the scaled ones are copies), so
@@ -201,7 +202,7 @@ def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.
     del ys
     tr, va = scaled[:-val], scaled[-val:]
     log(f"target scale {scale}")
-    net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid, resolution=resolution)
+    net = surrogate.MeshNet(seed=0, target_scale=scale, grid=grid, resolution=resolution, layers=layers)
     t2 = time.perf_counter()
     hist = surrogate.train(net, tr, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, log=log,
                            weights=weights)
@@ -212,7 +213,7 @@ def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.
     log(f"trained in {t_train:.1f} s; validation relative L2 (rho-1, ux, uy): "
         f"median {np.median(err, axis=0)}, max {err.max(axis=0)}")
     record = dict(train=ntr, val=nva, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final,
-                  loss=hist, grid=list(grid), resolution=resolution, loss_weights=list(weights),
+                  loss=hist, grid=list(grid), resolution=resolution, loss_weights=list(weights), layers=layers,
                   val_rel_l2_median=np.median(err, axis=0).tolist(), val_rel_l2_max=err.max(axis=0).tolist(),
                   train_s=t_train)
     return net, record
@@ -239,6 +240,7 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=16,
                     help="meshing processes (0: mesh in this process, for profiling)")
     ap.add_argument("--grid", default="32x16", help="virtual grid GXxGY (GY = 1: bands of x)")
+    ap.add_argument("--layers", type=int, default=surrogate.LAYERS, help="message-passing layers")
     ap.add_argument("--no-resolution", action="store_true",
                     help="leave out the mesh-resolution input column")
     ap.add_argument("--data-seconds", type=float, default=None,
@@ -259,7 +261,8 @@ def main(argv=None):
     log(f"data: {len(info)} steady states in {stats['elapsed_s']:.1f} s (solves {stats['solve_s']:.1f} s, "
         f"waiting for meshes {stats['mesh_wait_s']:.1f} s)")
     weights = tuple(float(v) for v in a.loss_weights.split(","))
-    net, rec = fit(samples[grid], a.val, grid, resolution, a.epochs, a.batch, a.lr, a.lr_final, weights, log)
+    net, rec = fit(samples[grid], a.val, grid, resolution, a.epochs, a.batch, a.lr, a.lr_final, weights, log,
+                   layers=a.layers)
     state = net.state()
     record = dict(cases=stats["attempted"], dist=a.dist, seed0=a.seed0, converged=len(info), tol=a.tol,
                   init_weights=os.path.basename(a.init_weights) if a.init_weights else None,
