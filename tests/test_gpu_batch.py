@@ -198,3 +198,18 @@ def test_fp64_batch_equals_single_solves_and_oracle(gpu):
     oracle.run(oenv, st, dyn.tau, dyn.delta_t, 61, "f64")
     assert np.array_equal(b.case_pdf(0, pdf), st["cell_pdf"])
     b.close()
+
+
+def test_lax_wendroff_batch_equals_single_solves(gpu):
+    """A batch of Lax-Wendroff cases (one step per launch, the scheme's
+    kernel) equals each case's single Lax-Wendroff solve bitwise."""
+    cs = [porous_case(seed, target_cells=5000 + 1000 * seed, scheme="lax_wendroff") for seed in range(3)]
+    b = batch.BatchSolver([(c, f, n) for _, _, c, f, n in cs])
+    assert b.info()["variant"] & 8  # Lax-Wendroff runs one step per launch
+    b.step(45)
+    pdf = b.get_pdf()
+    assert np.isfinite(pdf).all()
+    for k, (m, dyn, c, f, n) in enumerate(cs):
+        single, _ = _single(m, dyn, c, f, n, 45)
+        assert np.array_equal(b.case_pdf(k, pdf), single), k
+    b.close()
