@@ -64,6 +64,7 @@ def static_hash(env):
 def save_checkpoint(mesher, env, filename):
     out = dict(points=mesher.points, mesh_cells=mesher.cells, mesh_faces=mesher.faces,
                point_markers=mesher.point_markers,
+               lattice=np.array(type(env.cells.dynamics).__name__),
                tau=np.float64(env.cells.dynamics.tau), delta_t=np.float64(env.cells.dynamics.delta_t),
                flux_scheme=np.array(env.faces.flux_scheme), precision=np.array(env.precision),
                cell_pdf=env.cells.pdf, node_pdf=env.nodes.pdf,
@@ -90,7 +91,7 @@ def load_checkpoint(filename):
     """Returns (env, mesher): a new initialised Environment holding the saved
     pdfs, and the Mesher with the saved input arrays."""
     from .containers import Cells, Faces, Nodes
-    from .dynamics import D2Q9
+    from .dynamics import D2Q9, D2Q13
     from .environment import Environment
     from .mesher import Mesher
 
@@ -98,7 +99,11 @@ def load_checkpoint(filename):
     m = Mesher()
     m.points, m.cells, m.faces, m.point_markers = (z["points"], z["mesh_cells"], z["mesh_faces"],
                                                    z["point_markers"])
-    dyn = D2Q9(float(z["tau"]), float(z["delta_t"]))
+    lattices = {"D2Q9": D2Q9, "D2Q13": D2Q13}
+    name = str(z["lattice"]) if "lattice" in z.files else "D2Q9"  # files written before it was stored
+    if name not in lattices:
+        raise ValueError(f"checkpoint lattice {name!r} is not one of {sorted(lattices)}")
+    dyn = lattices[name](float(z["tau"]), float(z["delta_t"]))
     C, F, N = z["cells__face_indices"].shape[0], z["faces__nodes_index"].shape[0], z["nodes__type"].shape[0]
     cells, faces, nodes = Cells(C, dyn), Faces(F, dyn, str(z["flux_scheme"])), Nodes(N, dyn)
     for kind, obj in (("cells", cells), ("faces", faces), ("nodes", nodes)):

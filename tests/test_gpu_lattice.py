@@ -144,3 +144,23 @@ def test_lattice_handle_limits(gpu):
         env.cells.rho = np.zeros((cells.size, 1), np.float32)
     ms = env.time_steps(64)
     assert ms > 0
+
+
+def test_d2q13_checkpoint_resume_bitwise(gpu, tmp_path):
+    """to_pickle / from_pickle keep the lattice: 25 D2Q13 steps, a
+    checkpoint, a new environment from it and 35 more steps equal the D2Q13
+    oracle's 60 straight steps, every observable bitwise."""
+    m, dyn, cells, faces, nodes = _channel13(target_cells=6_000)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes, "f32", 13)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 60, "f32", quivers=13)
+    env = fv.Environment(cells, faces, nodes)
+    env.init()
+    env.step(25)
+    m.to_pickle(env, str(tmp_path / "ck"))
+    env.close()
+    env2, m2 = fv.Mesher.from_pickle(str(tmp_path / "ck"))
+    assert type(env2.cells.dynamics).__name__ == "D2Q13"
+    env2.step(35)
+    _equal(env2, st, "resumed D2Q13 run")
+    env2.close()
