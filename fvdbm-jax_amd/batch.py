@@ -71,6 +71,10 @@ class BatchSolver(_lib.FieldsMixin):
         schemes = {f.flux_scheme for _, f, _ in cases}
         if len(schemes) != 1:
             raise ValueError("all cases of a batch use one flux scheme")
+        lattices = {int(c.dynamics.NUM_QUIVERS) for c, _, _ in cases}
+        if lattices != {9}:
+            raise ValueError(f"batched solves run the D2Q9 lattice; these cases have {sorted(lattices)} "
+                             "velocities (a D2Q13 case runs as its own Environment)")
         self.cases = cases
         self.precision = precision
         self.dtype = np.float64 if precision == "fp64" else np.float32

@@ -1,10 +1,12 @@
 """Host logic of the porous workloads and the warm start, on the CPU:
 cross-section features against closed forms, the explicit time-step bound,
-case grouping for batched solves and the plausibility gate."""
+case grouping for batched solves, the plausibility gate and the batched
+solver's input checks."""
 import importlib
 import math
 
 import numpy as np
+import pytest
 
 from cases import fv
 
@@ -67,3 +69,18 @@ def test_plausibility_gate():
     v = vel.copy()
     v[1, 1] = np.inf
     assert not ws.plausible(rho, v)
+
+
+def test_batch_rejects_other_lattices_and_mixed_schemes():
+    """The batched kernels are D2Q9: a D2Q13 case is refused (not stepped with
+    D2Q9 arithmetic), as is a batch mixing flux schemes. Both checks come
+    before anything is uploaded, so no GPU is needed."""
+    from cases import ldc_case
+    batch = importlib.import_module("fvdbm-jax_amd.batch")
+    _, _, c13, f13, n13 = ldc_case(lattice=fv.D2Q13)
+    with pytest.raises(ValueError, match="D2Q9"):
+        batch.BatchSolver([(c13, f13, n13)])
+    _, _, c, f, n = ldc_case()
+    _, _, cl, fl, nl = ldc_case(scheme="lax_wendroff")
+    with pytest.raises(ValueError, match="one flux scheme"):
+        batch.BatchSolver([(c, f, n), (cl, fl, nl)])
