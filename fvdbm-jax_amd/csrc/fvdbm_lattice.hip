@@ -842,6 +842,9 @@ struct Handle {
   std::vector<void *> bufs;
 
   ~Handle() {
+    // steps are queued asynchronously: let them finish before their graphs,
+    // buffers and stream go
+    if (stream) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);

@@ -1365,6 +1365,10 @@ struct Handle {
   hipGraph_t graph[2] = {nullptr, nullptr};
 
   ~Handle() {
+    // steps are queued asynchronously: let them finish before their graphs,
+    // buffers and streams go
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (stream) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < 2; ++i) {
       if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
       if (graph[i]) (void)hipGraphDestroy(graph[i]);

@@ -282,3 +282,22 @@ def test_checkpoint_resume_bitwise(gpu, tmp_path):
     assert np.array_equal(m2.cells, m.cells)
     env2.step(35)
     _assert_fields_equal(env2, st, what="resumed run")
+
+
+def test_close_with_steps_still_queued(gpu):
+    """Steps are queued asynchronously; closing a handle right after a long
+    queued run lets it finish before its graphs and buffers go. A new handle
+    afterwards steps as the oracle does."""
+    m, dyn, cells, faces, nodes = ldc_case()
+    env = fv.Environment(cells, faces, nodes)
+    env.init()
+    env.step(3001)  # a graph replay of two-step launches plus one single step, not awaited
+    env.close()
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 12, "f32")
+    env2 = fv.Environment(cells, faces, nodes)
+    env2.init()
+    env2.step(12)
+    assert np.array_equal(env2.cells.pdf, st["cell_pdf"])
+    env2.close()
