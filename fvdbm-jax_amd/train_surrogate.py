@@ -193,6 +193,8 @@ def fit(samples, val, grid, resolution, epochs, batch, lr, lr_final, weights=(1.
     MeshNet on all but the last ``val`` samples, evaluate on those. The
     samples keep their unscaled targets (the scaled ones are copies), so
     they can train another net. Returns (net, record dict)."""
+    if not 1 <= val < len(samples):
+        raise ValueError(f"val must leave training samples: val={val}, {len(samples)} samples")
     torch = surrogate._torch()
     with torch.no_grad():
         ys = torch.cat([s["y"][s["mask"][:, 0] > 0] for s in samples[:-val]]).double()
