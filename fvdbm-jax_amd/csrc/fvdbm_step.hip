@@ -352,8 +352,12 @@ struct StepArgs {
   Lattice<T> k;
 };
 
-// Two-step tiles (k_step2): ring lists, ring face records and the extended
-// node tables (see HostLayout in layout.h).
+// Two-step tiles (k_step2): ring lists, face records, ghost lists and the
+// extended node tables (see HostLayout in layout.h). Every face record of
+// the two-step kernel names its neighbour by a stage slot: own cells 0..bs-1,
+// the tile's ring (S1 \ O, then S2) from bs, then one ghost record per
+// ghost face of S1 (the own cells' first, each tile's in boundary-face
+// order, then the S1 \ O cells').
 template <typename T>
 struct PairArgs {
   const int32_t *__restrict__ ring_off;    // [ntiles+1] into ring_cells
@@ -361,14 +365,17 @@ struct PairArgs {
   const int32_t *__restrict__ ring_n1;     // [ntiles] |S1 \ O|
   const int32_t *__restrict__ rg_off;      // [ntiles] first ring face record of the tile
   const void *rg_geo;                      // [3][nrg_cells] face records of S1 \ O cells
-  const int32_t *__restrict__ rgb_slot;    // [2][nrgb] extended-table slots
-  const T *__restrict__ rgb_ratio;         // [nrgb]
+  const void *geo2;                        // [3][Cp] face records of the own cells
+  const int32_t *__restrict__ gh_off;      // [ntiles+1] into the ghost lists
+  const int32_t *__restrict__ gh_nown;     // [ntiles] ghosts of own cells (listed first)
+  const int32_t *__restrict__ gh_s0, *__restrict__ gh_s1;  // extended-table slots of the face nodes
+  const int32_t *__restrict__ gh_self;     // stage slot of the cell owning the face
+  const T *__restrict__ gh_ratio;          // ghost distance / known distance
   const int32_t *__restrict__ xnode_off, *__restrict__ xnodes, *__restrict__ xentry_pair_off,
       *__restrict__ xpair_off, *__restrict__ xpair_cell;
   const T *__restrict__ xpair_w;
   const int32_t *__restrict__ pair_slot;   // stage slot of each own-table pair
   int nrg_cells;                           // ring cells with face records (all tiles)
-  int nrgb;
   int max_xnodes, max_xpairs;              // LDS partition: table rows, pair records
 };
 
@@ -731,17 +738,42 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<T> a) {
   if constexpr (RESID) block_reduce_store(res_d, res_n, a.partials + 2 * (size_t)tile);
 }
 
-// One upwind cell update with every neighbour in the LDS stage and the
-// ghost-face node values in the LDS node table (rows of length nrow): the
-// arithmetic of k_step<T, 0, ..> phase 2 with staging and inline nodes,
-// operation for operation.
+// Ghost records of one step (Faces.calc_ghost + utils.extrap_pdf,
+// src/containers.py:280-287, utils/utils.py:153-154): for ghost face i of
+// the list [g0, g0 + n), fbar = mean of its two node pdfs in the LDS node
+// table (rows of length nrow) and g = fbar + (fbar - f_self) * ratio, with
+// f_self the owning cell's record in the stage; g goes to stage slot
+// slot0 + i. One lane per ghost face, so the cell updates read every face
+// neighbour as a stage record and a wave never runs the extrapolation for
+// a few of its lanes.
+template <typename T>
+__device__ __forceinline__ void ghost_records(T *shf, const T *tab, int nrow,
+                                              const int32_t *__restrict__ s0v,
+                                              const int32_t *__restrict__ s1v,
+                                              const int32_t *__restrict__ selfv,
+                                              const T *__restrict__ ratiov, int g0, int n,
+                                              int slot0) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int s0 = s0v[g0 + i], s1 = s1v[g0 + i];
+    const T ratio = ratiov[g0 + i];
+    T fs[Q], g[Q];
+    LdsCell<T>::get(shf, selfv[g0 + i], fs);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
+      g[q] = fbar + (fbar - fs[q]) * ratio;
+    }
+    LdsCell<T>::put(shf, slot0 + i, g);
+  }
+}
+
+// One upwind cell update with every face neighbour, ghost records
+// included, in the LDS stage (code = stage slot | slot-1 bit): the
+// arithmetic of k_step<T, 0, ..> phase 2, operation for operation.
 template <typename T>
 __device__ __forceinline__ void cell_update_staged(const Lattice<T> &k, const T (&f)[Q],
                                                    const FaceRec<T> (&fr)[3], const T *shf,
-                                                   const T *tab, int nrow,
-                                                   const int32_t *__restrict__ gslot, int gstride,
-                                                   const T *__restrict__ gratio, T invtau, T dt,
-                                                   T (&fn)[Q]) {
+                                                   T invtau, T dt, T (&fn)[Q]) {
   T acc[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) acc[q] = T(0);
@@ -751,18 +783,7 @@ __device__ __forceinline__ void cell_update_staged(const Lattice<T> &k, const T 
     const T nx = fr[kf].nx, ny = fr[kf].ny, sL = fr[kf].sL;
     const bool slot1 = (code & kSlot1Bit) != 0;
     T o[Q];
-    if (code >= 0) {
-      LdsCell<T>::get(shf, code & kIndexMask, o);
-    } else {
-      const int bf = code & kIndexMask;
-      const T ratio = gratio[bf];
-      const int s0 = gslot[bf], s1 = gslot[gstride + bf];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
-        o[q] = fbar + (fbar - f[q]) * ratio;
-      }
-    }
+    LdsCell<T>::get(shf, code & kIndexMask, o);
     const T p5 = nx + ny, p6 = ny - nx;
     const T phi[Q] = {T(0), nx, ny, -nx, -ny, p5, p6, -p5, -p6};
 #pragma unroll
@@ -908,7 +929,7 @@ __global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
   FaceRec<T> fr[3] = {};
   if (live) {
     load_cell_at(fsrc, (size_t)a.Cp, c, f);
-    const Buf geo(a.geo, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
+    const Buf geo(p.geo2, 3 * (uint32_t)a.Cp * GeoLayout<T>::bytes);
     const uint32_t gplane = (uint32_t)a.Cp * GeoLayout<T>::bytes;
     const uint32_t gvo = (uint32_t)c * GeoLayout<T>::bytes;
 #pragma unroll
@@ -919,6 +940,9 @@ __global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
   }
   LdsCell<T>::put(shf, threadIdx.x, f);
   const int r0 = p.ring_off[tile], nr = p.ring_off[tile + 1] - r0, n1 = p.ring_n1[tile];
+  // ghost records: the own cells' ghost faces, then the S1 \ O cells', in
+  // stage slots after the ring
+  const int g0 = p.gh_off[tile], ng = p.gh_off[tile + 1] - g0, ngo = p.gh_nown[tile];
   for (int i = threadIdx.x; i < nr; i += bs) {
     T g[Q];
     load_cell_at(fsrc, (size_t)a.Cp, p.ring_cells[r0 + i], g);
@@ -935,29 +959,38 @@ __global__ void __launch_bounds__(1024) k_step2(StepArgs<T> a, PairArgs<T> p) {
     for (int kf = 0; kf < 3; ++kf) rr[kf] = GeoLayout<T>::load(rgeo, rvo, kf * rplane);
   }
   __syncthreads();
+  if (ng > 0) {  // uniform per block
+    ghost_records(shf, sh, nx, p.gh_s0, p.gh_s1, p.gh_self, p.gh_ratio, g0, ng, bs + nr);
+    __syncthreads();
+  }
   // step n for the own cell and the ring cell of this thread
   T fo[Q], fq[Q];
-  if (live)
-    cell_update_staged(a.k, f, fr, shf, sh, nx, a.bf_slot, a.nbf, a.bf_ratio, invtau, dt, fo);
+  if (live) cell_update_staged(a.k, f, fr, shf, invtau, dt, fo);
   if (ring_here) {
     T g[Q];
     LdsCell<T>::get(shf, bs + threadIdx.x, g);
-    cell_update_staged(a.k, g, rr, shf, sh, nx, p.rgb_slot, p.nrgb, p.rgb_ratio, invtau, dt, fq);
+    cell_update_staged(a.k, g, rr, shf, invtau, dt, fq);
   }
   __syncthreads();  // every step-n input has been read
   if (live) LdsCell<T>::put(shf, threadIdx.x, fo);
   if (ring_here) LdsCell<T>::put(shf, bs + threadIdx.x, fq);
   __syncthreads();
-  // step-(n+1) node values of the own ghost faces, from the stage
+  // step-(n+1) node values of the own ghost faces, from the stage, and the
+  // own cells' ghost records from them (the first ngo of the list, same
+  // slots)
   if (ntn > 0) {
     const int pb = a.tile_pair_off[tile];
     node_table<T, true>(a, sh, nx, con, a.tile_nodes + t0, ntn, a.entry_pair_off + t0, pb,
                         a.tile_pair_off[tile + 1] - pb, a.pair_cell, p.pair_slot, a.pair_w, fin,
                         plane, shf);
   }
+  if (ngo > 0) {  // uniform per block
+    ghost_records(shf, sh, nx, p.gh_s0, p.gh_s1, p.gh_self, p.gh_ratio, g0, ngo, bs + nr);
+    __syncthreads();
+  }
   if (live) {
     T fn[Q];
-    cell_update_staged(a.k, fo, fr, shf, sh, nx, a.bf_slot, a.nbf, a.bf_ratio, invtau, dt, fn);
+    cell_update_staged(a.k, fo, fr, shf, invtau, dt, fn);
 #pragma unroll
     for (int q = 0; q < Q; ++q) fdst[(size_t)q * a.Cp + c] = fn[q];
   }
@@ -1321,9 +1354,11 @@ struct Handle {
   int pair_grid[2] = {0, 0};
   size_t pair_lds[2] = {0, 0};
   int pair_xnodes[2] = {0, 0}, pair_xpairs[2] = {0, 0};
-  int nrg_cells = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0;
+  int nrg_cells = 0, max_xnodes = 0, max_xpairs = 0;
   void *ring_off = nullptr, *ring_cells = nullptr, *ring_n1 = nullptr, *rg_off = nullptr,
-       *rg_geo = nullptr, *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr,
+       *rg_geo = nullptr, *geo2 = nullptr, *gh_off = nullptr, *gh_nown = nullptr,
+       *gh_s0 = nullptr, *gh_s1 = nullptr, *gh_self = nullptr, *gh_ratio = nullptr,
+       *xnode_off = nullptr,
        *xnodes = nullptr, *xentry_pair_off = nullptr, *xpair_off = nullptr,
        *xpair_cell = nullptr, *xpair_w = nullptr, *pair_slot = nullptr;
   // captured runs of two-step launches, keyed by (starting parity, launches)
@@ -1523,8 +1558,13 @@ static int launch_pair_typed(Handle &h, int src) {
   p.ring_n1 = static_cast<const int32_t *>(h.ring_n1);
   p.rg_off = static_cast<const int32_t *>(h.rg_off);
   p.rg_geo = h.rg_geo;
-  p.rgb_slot = static_cast<const int32_t *>(h.rgb_slot);
-  p.rgb_ratio = static_cast<const T *>(h.rgb_ratio);
+  p.geo2 = h.geo2;
+  p.gh_off = static_cast<const int32_t *>(h.gh_off);
+  p.gh_nown = static_cast<const int32_t *>(h.gh_nown);
+  p.gh_s0 = static_cast<const int32_t *>(h.gh_s0);
+  p.gh_s1 = static_cast<const int32_t *>(h.gh_s1);
+  p.gh_self = static_cast<const int32_t *>(h.gh_self);
+  p.gh_ratio = static_cast<const T *>(h.gh_ratio);
   p.xnode_off = static_cast<const int32_t *>(h.xnode_off);
   p.xnodes = static_cast<const int32_t *>(h.xnodes);
   p.xentry_pair_off = static_cast<const int32_t *>(h.xentry_pair_off);
@@ -1533,7 +1573,6 @@ static int launch_pair_typed(Handle &h, int src) {
   p.xpair_w = static_cast<const T *>(h.xpair_w);
   p.pair_slot = static_cast<const int32_t *>(h.pair_slot);
   p.nrg_cells = h.nrg_cells;
-  p.nrgb = h.nrgb;
   p.max_xnodes = h.max_xnodes;
   p.max_xpairs = h.max_xpairs;
   if (h.npair_class == 0) {
@@ -1849,16 +1888,17 @@ static int run_steps(Handle &h, int64_t n, bool resid_last, bool dry = false) {
 
 // LDS of a two-step launch over a set of tiles: the kernel partitions it
 // with the set's largest node table (rows), largest pair count (extended or
-// own table) and largest ring: [9][rows] table, pair records rounded to 8
-// words, then the stage of block + ring cell records.
+// own table) and largest ring plus ghost records: [9][rows] table, pair
+// records rounded to 8 words, then the stage of block + ring + ghost cell
+// records.
 struct PairLds {
   int rows = 0, pairs = 0, ring = 0;
   template <typename T>
-  void add(const HostLayout<T> &L, int t) {
+  void add(const HostLayout<T> &L, int t, int ghosts) {
     rows = std::max(rows, L.xnode_off[t + 1] - L.xnode_off[t]);
     pairs = std::max(pairs, std::max(L.xpair_off[t + 1] - L.xpair_off[t],
                                      L.tile_pair_off[t + 1] - L.tile_pair_off[t]));
-    ring = std::max(ring, L.ring_off[t + 1] - L.ring_off[t]);
+    ring = std::max(ring, L.ring_off[t + 1] - L.ring_off[t] + ghosts);
   }
   size_t bytes(int block, size_t es) const {
     const size_t pre = ((size_t)Q * rows + kPairW * (size_t)pairs + 7) & ~(size_t)7;
@@ -1874,7 +1914,8 @@ struct PairLds {
 // Each class keeps the XCD-aware order (boundary tiles first per range).
 // Without occupancy information the single launch stays.
 template <typename T>
-static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
+static int build_pair_classes(Handle &h, const HostLayout<T> &L,
+                              const std::vector<int32_t> &ghosts) {
   int dev = 0, lds_cu = 0, nb = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
@@ -1891,7 +1932,7 @@ static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
   std::vector<size_t> need(L.ntiles);
   for (int t = 0; t < L.ntiles; ++t) {
     PairLds one;
-    one.add(L, t);
+    one.add(L, t, ghosts[t]);
     need[t] = one.bytes(h.block, es);
   }
   // the light class: tiles up to a threshold, lowered until the class's
@@ -1905,7 +1946,7 @@ static int build_pair_classes(Handle &h, const HostLayout<T> &L) {
     n_light = 0;
     for (int t = 0; t < L.ntiles; ++t) {
       const int k = need[t] <= thr ? 0 : 1;
-      cls[k].add(L, t);
+      cls[k].add(L, t, ghosts[t]);
       n_light += k == 0;
     }
     if (cls[0].bytes(h.block, es) <= budget || thr < 1024) break;
@@ -2116,32 +2157,90 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
     if (!rc) rc = h.upload(&h.cn_d, cn_d);
   }
   if (!rc && !(h.variant & 8) && h.scheme == FVDBM_UPWIND && !(h.variant & 6) && L.pair_ok) {
-    // two-step tiles, when their LDS fits a block
+    // two-step tiles, when their LDS fits a block. Ghost lists: per tile the
+    // ghost faces of the own cells (device order, faces 0..2: ascending
+    // boundary-face index), then those of the S1 \ O cells (ring order);
+    // ghost j of tile t is staged at slot block + |ring of t| + j, and the
+    // two-step face records name it there instead of by its ghost bit.
+    const int bs = h.block;
+    const size_t Cp = (size_t)L.Cp;
+    std::vector<int32_t> gh_off(L.ntiles + 1, 0), gh_nown(L.ntiles, 0), gh_s0, gh_s1, gh_self;
+    std::vector<T> gh_ratio;
+    std::vector<int32_t> code2(L.nb_slot), rgc(L.rg_code), gh_count(L.ntiles, 0);
+    int max_stage = 0;  // largest ring + ghost records of a tile
+    for (int t = 0, rc0 = 0; t < L.ntiles; ++t) {
+      const int nr = L.ring_off[t + 1] - L.ring_off[t], n1 = L.ring_n1[t];
+      gh_off[t] = (int32_t)gh_s0.size();
+      int j = 0;
+      for (int i = t * bs; i < t * bs + L.tile_count[t]; ++i)
+        for (int k = 0; k < 3; ++k) {
+          const size_t at = (size_t)k * Cp + i;
+          const int32_t code = L.nb_slot[at];
+          if (code >= 0) continue;
+          const int bf = code & kIndexMask;
+          gh_s0.push_back(L.bf_slot[bf]);
+          gh_s1.push_back(L.bf_slot[(size_t)L.nbf + bf]);
+          gh_ratio.push_back(L.bf_ratio[bf]);
+          gh_self.push_back(i - t * bs);
+          code2[at] = (code & kSlot1Bit) | (bs + nr + j);
+          ++j;
+        }
+      gh_nown[t] = j;
+      for (int r = 0; r < n1; ++r, ++rc0)
+        for (int k = 0; k < 3; ++k) {
+          const size_t si = 3 * (size_t)rc0 + k;
+          const int32_t code = L.rg_code[si];
+          if (code >= 0) continue;
+          const int b = code & kIndexMask;
+          gh_s0.push_back(L.rgb_slot[b]);
+          gh_s1.push_back(L.rgb_slot[(size_t)L.nrg + b]);
+          gh_ratio.push_back(L.rgb_ratio[b]);
+          gh_self.push_back(bs + r);
+          rgc[si] = (code & kSlot1Bit) | (bs + nr + j);
+          ++j;
+        }
+      gh_count[t] = j;
+      max_stage = std::max(max_stage, nr + j);
+    }
+    gh_off[L.ntiles] = (int32_t)gh_s0.size();
     const size_t pre2 = ((size_t)Q * L.max_xnodes + kPairW * (size_t)L.max_xpairs + 7) & ~(size_t)7;
-    const size_t lds2 = (pre2 + (size_t)9 * (h.block + L.max_ring)) * es;
+    const size_t lds2 = (pre2 + (size_t)9 * (bs + max_stage)) * es;
     if (lds2 <= 65536) {
       const int nrgc = (int)(L.rg_code.size() / 3);
       const size_t rec = GeoLayout<T>::bytes;
+      const size_t o = sizeof(T) == 4 ? 4 : 8;
       std::vector<unsigned char> g(3 * (size_t)std::max(nrgc, 1) * rec, 0);
       for (int r = 0; r < nrgc; ++r)
         for (int k = 0; k < 3; ++k) {
           unsigned char *d = g.data() + ((size_t)k * nrgc + r) * rec;
           const size_t si = 3 * (size_t)r + k;
-          std::memcpy(d, &L.rg_code[si], 4);
-          const size_t o = sizeof(T) == 4 ? 4 : 8;
+          std::memcpy(d, &rgc[si], 4);
           std::memcpy(d + o, &L.rg_nx[si], sizeof(T));
           std::memcpy(d + o + sizeof(T), &L.rg_ny[si], sizeof(T));
           std::memcpy(d + o + 2 * sizeof(T), &L.rg_sL[si], sizeof(T));
         }
+      std::vector<unsigned char> g2(3 * Cp * rec, 0);
+      for (size_t i = 0; i < 3 * Cp; ++i) {
+        unsigned char *d = g2.data() + i * rec;
+        std::memcpy(d, &code2[i], 4);
+        std::memcpy(d + o, &L.nx[i], sizeof(T));
+        std::memcpy(d + o + sizeof(T), &L.ny[i], sizeof(T));
+        std::memcpy(d + o + 2 * sizeof(T), &L.sL[i], sizeof(T));
+      }
       std::vector<int32_t> rg_off(L.ntiles, 0);
       for (int t = 1; t < L.ntiles; ++t) rg_off[t] = rg_off[t - 1] + L.ring_n1[t - 1];
       rc = h.upload(&h.rg_geo, g);
+      if (!rc) rc = h.upload(&h.geo2, g2);
       if (!rc) rc = h.upload(&h.rg_off, rg_off);
       if (!rc) rc = h.upload(&h.ring_off, L.ring_off);
       if (!rc) rc = h.upload(&h.ring_cells, L.ring_cells);
       if (!rc) rc = h.upload(&h.ring_n1, L.ring_n1);
-      if (!rc) rc = h.upload(&h.rgb_slot, L.rgb_slot);
-      if (!rc) rc = h.upload(&h.rgb_ratio, L.rgb_ratio);
+      if (!rc) rc = h.upload(&h.gh_off, gh_off);
+      if (!rc) rc = h.upload(&h.gh_nown, gh_nown);
+      if (!rc) rc = h.upload(&h.gh_s0, gh_s0);
+      if (!rc) rc = h.upload(&h.gh_s1, gh_s1);
+      if (!rc) rc = h.upload(&h.gh_self, gh_self);
+      if (!rc) rc = h.upload(&h.gh_ratio, gh_ratio);
       if (!rc) rc = h.upload(&h.xnode_off, L.xnode_off);
       if (!rc) rc = h.upload(&h.xnodes, L.xnodes);
       if (!rc) rc = h.upload(&h.xentry_pair_off, L.xentry_pair_off);
@@ -2154,11 +2253,10 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, const int32_t *cell
         h.pairs = true;
         h.lds2 = lds2;
         h.nrg_cells = nrgc;
-        h.nrgb = L.nrg;
         h.max_xnodes = L.max_xnodes;
         h.max_xpairs = L.max_xpairs;
       }
-      if (!rc && !(h.variant & 1)) rc = build_pair_classes<T>(h, L);
+      if (!rc && !(h.variant & 1)) rc = build_pair_classes<T>(h, L, gh_count);
     }
   }
   if (!rc && !h.pairs) h.variant |= 8;  // one step per launch (two-step tiles off or unavailable)
