@@ -203,14 +203,16 @@ def batched(triples, steps=1000, masked_steps=700_000, check_every=2000, tune_ca
     until its residual is below 1e-6 (converged), its mean density passes 2
     (runaway: a choked channel has no steady state) or ``masked_steps`` have
     run. The launch options are tuned on the first ``tune_cases`` cases
-    (every candidate checked bitwise), or fixed to tile size ``block`` with
-    two-step tiles. FP32 against FP64 on the first ``fp64_cases`` cases after
+    (every candidate checked bitwise) and the best candidates of the two
+    fastest tile sizes timed again on the whole batch, or fixed to tile size
+    ``block`` with two-step tiles. FP32 against FP64 on the first ``fp64_cases`` cases after
     ``fp64_steps`` steps (per-case medians; 0 cases: skipped)."""
     batch = importlib.import_module("fvdbm-jax_amd.batch")
     ws = importlib.import_module("fvdbm-jax_amd.warmstart")
     _lib = importlib.import_module("fvdbm-jax_amd._lib")
     import ctypes as ct
     at = importlib.import_module("fvdbm-jax_amd.autotune")
+    b, confirmed = None, None
     if block:
         best, default = dict(block=int(block), variant=0, reorder=4), []
     else:
@@ -218,11 +220,18 @@ def batched(triples, steps=1000, masked_steps=700_000, check_every=2000, tune_ca
                               cands=at.candidates(blocks=(128, 256, 512), variants=(0, 8, 2, 4),
                                                   reorders=(4,)),
                               steps=200, cache=None, log=log)
-        best = tuned["best"]
         default = [r for r in tuned["results"] if (r["block"], r["variant"]) == (256, 0)]
-    t0 = time.perf_counter()
-    b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
-    log(f"batched handle: {len(triples)} cases uploaded in {time.perf_counter() - t0:.1f}s")
+        if len(triples) > tune_cases:
+            # the two best tile sizes of the sample, timed on the whole batch
+            t0 = time.perf_counter()
+            best, confirmed, b = at.confirm_batch(triples, tuned["results"], log=log)
+            log(f"batched handles: {len(triples)} cases, two tile sizes in {time.perf_counter() - t0:.1f}s")
+        else:
+            best = tuned["best"]
+    if b is None:
+        t0 = time.perf_counter()
+        b = batch.BatchSolver(triples, block=best["block"], variant=best["variant"], reorder=best["reorder"])
+        log(f"batched handle: {len(triples)} cases uploaded in {time.perf_counter() - t0:.1f}s")
     cells = int(sum(c.face_indices.shape[0] for c, _, _ in triples))
     info = b.info()
     b.step(100)  # queued: the timed run below starts behind it
@@ -242,7 +251,8 @@ def batched(triples, steps=1000, masked_steps=700_000, check_every=2000, tune_ca
     return dict(cases=len(triples), cells=cells, padded_cells=int(info["n_cells_padded"]), steps=steps,
                 launch_classes=int(info["launch_classes"]),
                 tuned=dict(block=best["block"], variant=best["variant"], reorder=best["reorder"],
-                           on_cases=0 if block else min(tune_cases, len(triples))),
+                           on_cases=0 if block else min(tune_cases, len(triples)),
+                           full_batch=confirmed),
                 default_us_per_step=default[0]["us_per_step"] if default else None,
                 us_per_step=round(per * 1e6, 3), mlups=round(mlups, 1),
                 hbm_frac=round(BYTES_PER_LUP_F32 * cells / per / 1e9 / HBM_PEAK_GBS, 4),

@@ -149,3 +149,40 @@ def tune_batch(cases, precision="fp32", cands=None, steps=200, check_steps=20, c
     rec = dict(best=best, results=results)
     _store(cache, key, rec)
     return dict(rec, cached=False)
+
+
+def confirm_batch(cases, results, precision="fp32", top=2, steps=100, log=None):
+    """Time the best candidate of each of the ``top`` fastest tile sizes in
+    ``results`` (``tune_batch`` on a sample of ``cases``) on the whole batch
+    and keep the fastest. A two-step launch sizes its LDS classes from the
+    largest tables, rings and ghost lists of all its tiles, so a sample can
+    rank tile sizes differently from the full batch. Returns (best options,
+    full-batch timings, the winning BatchSolver, left open and stepped)."""
+    from .batch import BatchSolver
+    finalists = []
+    for r in sorted(results, key=lambda r: r["us_per_step"]):
+        if all(f["block"] != r["block"] for f in finalists):
+            finalists.append(dict(block=r["block"], variant=r["variant"], reorder=r["reorder"]))
+        if len(finalists) == top:
+            break
+    if not finalists:
+        raise ValueError("no tuning results to confirm")
+    keep, timings = None, []
+    for c in finalists:
+        b = BatchSolver(cases, precision=precision, **c)
+        try:
+            b.step(20)
+            us = 1e3 * _time(b, steps) / steps
+        except Exception:
+            b.close()
+            raise
+        timings.append(dict(c, us_per_step=round(us, 3)))
+        if log:
+            log(f"autotune on the full batch {c}: {us:.2f} us/step")
+        if keep is None or us < keep[0]:
+            if keep is not None:
+                keep[2].close()
+            keep = (us, c, b)
+        else:
+            b.close()
+    return keep[1], timings, keep[2]

@@ -39,3 +39,21 @@ def test_tune_batch(gpu, tmp_path):
     cands = at.candidates(blocks=(128, 256), variants=(0, 2), reorders=(4,))
     r = at.tune_batch(cases, cands=cands, steps=50, cache=str(tmp_path / "t.json"))
     assert len(r["results"]) == 4 and r["best"]["block"] in (128, 256)
+
+
+def test_confirm_batch_on_the_whole_batch(gpu):
+    cases = [fv.porous.make_case(s, target_cells=4000).env_tuple() for s in (41, 42, 43)]
+    cands = at.candidates(blocks=(128, 256), variants=(0,), reorders=(4,))
+    r = at.tune_batch(cases[:2], cands=cands, steps=50, cache=None)
+    best, timings, b = at.confirm_batch(cases, r["results"], steps=40)
+    try:
+        assert sorted(t["block"] for t in timings) == [128, 256]
+        assert best["block"] == min(timings, key=lambda t: t["us_per_step"])["block"]
+        # the kept handle has run 20 + 40 steps; a fresh one of the default
+        # options stepped as far gives the same bits
+        ref = importlib.import_module("fvdbm-jax_amd.batch").BatchSolver(cases)
+        ref.step(60)
+        assert np.array_equal(b.get_pdf(), ref.get_pdf())
+        ref.close()
+    finally:
+        b.close()
