@@ -1493,6 +1493,9 @@ int fvdbm_lattice_create(const fvdbm_mesh_desc *mesh, const fvdbm_params *p, int
     return create_typed<T, decltype(qn)::value>(*mesh, *h, reorder);
   });
   if (rc) return rc;
+  // default launch mode as for the tuned handle: plain launches from 128k
+  // padded cells, where a launch outlasts the host's queueing of the next
+  if (p->graph_steps == 0 && h->Cp >= 131072) h->graph_steps = -1;
   *out = h.release();
   return FVDBM_OK;
 }

@@ -96,7 +96,9 @@ typedef struct fvdbm_params {
                          advance two steps per launch where the tiles fit; the
                          handle's info reports bit 8 when they do not) */
   int32_t block;      /* threads per block (64..1024, multiple of 64), 0 = default */
-  int32_t graph_steps;/* steps per captured hipGraph, 0 = default, <0 = no graphs */
+  int32_t graph_steps;/* steps per captured hipGraph, <0 = no graphs, 0 = default: 32 below
+                         131072 padded cells, plain launches from there (single handles;
+                         fvdbm_info reports the mode chosen) */
 } fvdbm_params;
 
 typedef struct fvdbm_info {
