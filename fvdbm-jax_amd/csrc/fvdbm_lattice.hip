@@ -460,19 +460,24 @@ __global__ void __launch_bounds__(1024) k_lat_step(Args<T> a, Consts<T, QN> k) {
 // for S1 from the inputs of S1 and S2 staged in LDS, the results stay in
 // LDS, then step n + 1 runs for O. Every value equals what two k_lat_step
 // launches compute.
+// Every face code of the two-step kernel is a stage slot (| slot-1 bit):
+// own cells 0..bs-1, the tile's ring from bs, then one ghost record per
+// ghost face of S1 (the own cells' first, in boundary-face order, then the
+// S1 \ O cells').
 template <typename T>
 struct PArgs {
   const int32_t *code2;                    // [3][Cp] own face codes with stage slots
   const int32_t *ring_off, *ring_cells, *ring_n1, *rg_off;
   const int32_t *rg_code;                  // [3][nrgc] face records of the S1 \ O cells
   const T *rg_nx, *rg_ny, *rg_sL;
-  const int32_t *rgb_slot;                 // [2][nrgb] extended-table slots of ring ghosts
-  const T *rgb_ratio;
+  const int32_t *gh_off, *gh_nown;         // [ntiles+1] ghost lists; [ntiles] own ghosts (first)
+  const int32_t *gh_s0, *gh_s1, *gh_self;  // table slots of the face nodes; owning cell's slot
+  const T *gh_ratio;                       // ghost distance / known distance
   const int32_t *xnode_off, *xnodes, *xentry_pair_off, *xpair_off, *xpair_cell;
   const T *xpair_w;
   const int32_t *entry_pair_off, *tile_pair_off, *pair_cell, *pair_slot;
   const T *pair_w;
-  int nrgc, nrgb, max_xnodes, max_xpairs;
+  int nrgc, max_xnodes, max_xpairs;
 };
 
 // Node table entries [0, nent) into tab (rows of length nrow) from their
@@ -566,15 +571,33 @@ __device__ __forceinline__ void lat_table_rec(const Args<T> &a, const Consts<T, 
   __syncthreads();
 }
 
-// One upwind update of a cell whose neighbours are in the stage ([slot][QN])
-// and whose ghost-face nodes are in tab: k_lat_step's arithmetic.
+// Ghost records of one step (Faces.calc_ghost + utils.extrap_pdf,
+// src/containers.py:280-287, utils/utils.py:153-154): ghost face i of
+// [g0, g0 + n) gets fbar = mean of its two node rows in tab (rows of
+// length nrow) and g = fbar + (fbar - f_self) * ratio, with f_self the
+// owning cell's stage record, stored at stage slot slot0 + i. One lane per
+// ghost face, so the cell updates read every face neighbour from the stage.
+template <typename T, int QN>
+__device__ __forceinline__ void lat_ghost_records(T *stage, const T *tab, int nrow,
+                                                  const PArgs<T> &p, int g0, int n, int slot0) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int s0 = p.gh_s0[g0 + i], s1 = p.gh_s1[g0 + i], self = p.gh_self[g0 + i];
+    const T ratio = p.gh_ratio[g0 + i];
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
+      stage[(slot0 + i) * QN + q] = fbar + (fbar - stage[self * QN + q]) * ratio;
+    }
+  }
+}
+
+// One upwind update of a cell whose face neighbours, ghost records
+// included, are in the stage ([slot][QN]): k_lat_step's arithmetic.
 template <typename T, int QN>
 __device__ __forceinline__ void lat_update_staged(const Consts<T, QN> &k, const T (&f)[QN],
                                                   const int32_t (&code)[3], const T (&nx)[3],
                                                   const T (&ny)[3], const T (&sL)[3],
-                                                  const T *stage, const T *tab, int nrow,
-                                                  const int32_t *__restrict__ gslot, int gstride,
-                                                  const T *__restrict__ gratio, T (&fn)[QN]) {
+                                                  const T *stage, T (&fn)[QN]) {
   T acc[QN];
 #pragma unroll
   for (int q = 0; q < QN; ++q) acc[q] = T(0);
@@ -582,19 +605,10 @@ __device__ __forceinline__ void lat_update_staged(const Consts<T, QN> &k, const 
   for (int kf = 0; kf < 3; ++kf) {
     const bool slot1 = (code[kf] & kSlot1Bit) != 0;
     T o[QN];
-    if (code[kf] >= 0) {
+    {
       const int sl = code[kf] & kIndexMask;
 #pragma unroll
       for (int q = 0; q < QN; ++q) o[q] = stage[sl * QN + q];
-    } else {
-      const int bf = code[kf] & kIndexMask;
-      const T ratio = gratio[bf];
-      const int s0 = gslot[bf], s1 = gslot[gstride + bf];
-#pragma unroll
-      for (int q = 0; q < QN; ++q) {
-        const T fbar = (tab[q * nrow + s0] + tab[q * nrow + s1]) / T(2);
-        o[q] = fbar + (fbar - f[q]) * ratio;
-      }
     }
     T phi[QN];
     upwind_phi<T, QN>(nx[kf], ny[kf], phi);
@@ -659,6 +673,7 @@ __global__ void __launch_bounds__(1024) k_lat_step2(Args<T> a, PArgs<T> p, Const
 #pragma unroll
     for (int q = 0; q < QN; ++q) stage[(bs + i) * QN + q] = g[q];
   }
+  const int g0 = p.gh_off[tile], ng = p.gh_off[tile + 1] - g0, ngo = p.gh_nown[tile];
   const bool ring_here = (int)threadIdx.x < n1;
   int32_t rc[3] = {0, 0, 0};
   T rnx[3] = {}, rny[3] = {}, rsl[3] = {};
@@ -674,17 +689,18 @@ __global__ void __launch_bounds__(1024) k_lat_step2(Args<T> a, PArgs<T> p, Const
     }
   }
   __syncthreads();
+  if (ng > 0) {  // uniform per block
+    lat_ghost_records<T, QN>(stage, tab, nx, p, g0, ng, bs + nr);
+    __syncthreads();
+  }
   // step n for the own cell and the ring cell of this thread
   T fo[QN], fq[QN];
-  if (live)
-    lat_update_staged<T, QN>(k, f, cd, fnx, fny, fsl, stage, tab, nx, a.bf_slot, a.nbf, a.bf_ratio,
-                             fo);
+  if (live) lat_update_staged<T, QN>(k, f, cd, fnx, fny, fsl, stage, fo);
   if (ring_here) {
     T g[QN];
 #pragma unroll
     for (int q = 0; q < QN; ++q) g[q] = stage[(bs + threadIdx.x) * QN + q];
-    lat_update_staged<T, QN>(k, g, rc, rnx, rny, rsl, stage, tab, nx, p.rgb_slot, p.nrgb,
-                             p.rgb_ratio, fq);
+    lat_update_staged<T, QN>(k, g, rc, rnx, rny, rsl, stage, fq);
   }
   __syncthreads();  // every step-n input has been read
   if (live) {
@@ -703,10 +719,13 @@ __global__ void __launch_bounds__(1024) k_lat_step2(Args<T> a, PArgs<T> p, Const
                                pb, p.tile_pair_off[tile + 1] - pb, p.pair_cell, p.pair_slot,
                                p.pair_w, stage);
   }
+  if (ngo > 0) {  // the own cells' ghost records from the step-(n+1) table
+    lat_ghost_records<T, QN>(stage, tab, nx, p, g0, ngo, bs + nr);
+    __syncthreads();
+  }
   if (live) {
     T fn[QN];
-    lat_update_staged<T, QN>(k, fo, cd, fnx, fny, fsl, stage, tab, nx, a.bf_slot, a.nbf, a.bf_ratio,
-                             fn);
+    lat_update_staged<T, QN>(k, fo, cd, fnx, fny, fsl, stage, fn);
 #pragma unroll
     for (int q = 0; q < QN; ++q) a.fdst[(size_t)q * Cp + c] = fn[q];
   }
@@ -817,10 +836,11 @@ struct Handle {
   // two-step tiles (upwind, when the layout's rings fit and the LDS does)
   bool pairs = false, two_step = true;  // two_step: wanted (params.variant bit 8 clear)
   size_t lds2 = 0;
-  int nrgc = 0, nrgb = 0, max_xnodes = 0, max_xpairs = 0;
+  int nrgc = 0, max_xnodes = 0, max_xpairs = 0;
   void *code2 = nullptr, *ring_off = nullptr, *ring_cells = nullptr, *ring_n1 = nullptr,
        *rg_off = nullptr, *rg_code = nullptr, *rg_nx = nullptr, *rg_ny = nullptr, *rg_sL = nullptr,
-       *rgb_slot = nullptr, *rgb_ratio = nullptr, *xnode_off = nullptr, *xnodes = nullptr,
+       *gh_off = nullptr, *gh_nown = nullptr, *gh_s0 = nullptr, *gh_s1 = nullptr,
+       *gh_self = nullptr, *gh_ratio = nullptr, *xnode_off = nullptr, *xnodes = nullptr,
        *xentry_pair_off = nullptr, *xpair_off = nullptr, *xpair_cell = nullptr, *xpair_w = nullptr,
        *entry_pair_off = nullptr, *tile_pair_off = nullptr, *pair_cell = nullptr,
        *pair_slot = nullptr, *pair_w = nullptr;
@@ -962,8 +982,12 @@ static int launch_pair(Handle &h, int src) {
   p.rg_nx = static_cast<const T *>(h.rg_nx);
   p.rg_ny = static_cast<const T *>(h.rg_ny);
   p.rg_sL = static_cast<const T *>(h.rg_sL);
-  p.rgb_slot = static_cast<const int32_t *>(h.rgb_slot);
-  p.rgb_ratio = static_cast<const T *>(h.rgb_ratio);
+  p.gh_off = static_cast<const int32_t *>(h.gh_off);
+  p.gh_nown = static_cast<const int32_t *>(h.gh_nown);
+  p.gh_s0 = static_cast<const int32_t *>(h.gh_s0);
+  p.gh_s1 = static_cast<const int32_t *>(h.gh_s1);
+  p.gh_self = static_cast<const int32_t *>(h.gh_self);
+  p.gh_ratio = static_cast<const T *>(h.gh_ratio);
   p.xnode_off = static_cast<const int32_t *>(h.xnode_off);
   p.xnodes = static_cast<const int32_t *>(h.xnodes);
   p.xentry_pair_off = static_cast<const int32_t *>(h.xentry_pair_off);
@@ -976,7 +1000,6 @@ static int launch_pair(Handle &h, int src) {
   p.pair_slot = static_cast<const int32_t *>(h.pair_slot);
   p.pair_w = static_cast<const T *>(h.pair_w);
   p.nrgc = h.nrgc;
-  p.nrgb = h.nrgb;
   p.max_xnodes = h.max_xnodes;
   p.max_xpairs = h.max_xpairs;
   hipLaunchKernelGGL((k_lat_step2<T, QN>), dim3((unsigned)h.grid), dim3(h.block), h.lds2, h.stream,
@@ -1207,24 +1230,68 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
     rc = h.upload(&h.block_tile, bt);
   }
   if (rc) return rc;
-  // two-step tiles: upwind, rings within a tile's threads, LDS of a block
+  // two-step tiles: upwind, rings within a tile's threads, LDS of a block.
+  // Ghost lists: per tile the ghost faces of the own cells (device order,
+  // faces 0..2: ascending boundary-face index), then those of the S1 \ O
+  // cells (ring order); ghost j of tile t is staged at slot block + |ring
+  // of t| + j, and the two-step face codes name it there.
+  const int bs = h.block;
+  const size_t Cp = (size_t)L.Cp;
+  const int nrgc = (int)(L.rg_code.size() / 3);
+  std::vector<int32_t> gh_off(L.ntiles + 1, 0), gh_nown(L.ntiles, 0), gh_s0, gh_s1, gh_self;
+  std::vector<T> gh_ratio;
+  std::vector<int32_t> code2(L.nb_slot), rgc(L.rg_code);
+  int max_stage = 0;
+  for (int t = 0, r0 = 0; t < L.ntiles; ++t) {
+    const int nr = L.ring_off[t + 1] - L.ring_off[t], n1 = L.ring_n1[t];
+    gh_off[t] = (int32_t)gh_s0.size();
+    int j = 0;
+    for (int i = t * bs; i < t * bs + L.tile_count[t]; ++i)
+      for (int kf = 0; kf < 3; ++kf) {
+        const size_t at = (size_t)kf * Cp + i;
+        const int32_t code = L.nb_slot[at];
+        if (code >= 0) continue;
+        const int bf = code & kIndexMask;
+        gh_s0.push_back(L.bf_slot[bf]);
+        gh_s1.push_back(L.bf_slot[(size_t)L.nbf + bf]);
+        gh_ratio.push_back(L.bf_ratio[bf]);
+        gh_self.push_back(i - t * bs);
+        code2[at] = (code & kSlot1Bit) | (bs + nr + j);
+        ++j;
+      }
+    gh_nown[t] = j;
+    for (int r = 0; r < n1; ++r, ++r0)
+      for (int kf = 0; kf < 3; ++kf) {
+        const size_t si = 3 * (size_t)r0 + kf;
+        const int32_t code = L.rg_code[si];
+        if (code >= 0) continue;
+        const int b = code & kIndexMask;
+        gh_s0.push_back(L.rgb_slot[b]);
+        gh_s1.push_back(L.rgb_slot[(size_t)L.nrg + b]);
+        gh_ratio.push_back(L.rgb_ratio[b]);
+        gh_self.push_back(bs + r);
+        rgc[si] = (code & kSlot1Bit) | (bs + nr + j);
+        ++j;
+      }
+    max_stage = std::max(max_stage, nr + j);
+  }
+  gh_off[L.ntiles] = (int32_t)gh_s0.size();
   const size_t lds2 = ((size_t)QN * L.max_xnodes + (size_t)PairRec<QN>::W * L.max_xpairs +
-                       (size_t)QN * (h.block + L.max_ring)) * es;
+                       (size_t)QN * (bs + max_stage)) * es;
   if (h.two_step && h.scheme == FVDBM_UPWIND && L.pair_ok && lds2 <= 65536) {
-    const int nrgc = (int)(L.rg_code.size() / 3);
     std::vector<int32_t> rcode(3 * (size_t)std::max(nrgc, 1), 0);
     std::vector<T> rnx(rcode.size(), T(0)), rny(rcode.size(), T(0)), rsl(rcode.size(), T(0));
     for (int r = 0; r < nrgc; ++r)
       for (int kf = 0; kf < 3; ++kf) {
         const size_t d = (size_t)kf * nrgc + r, src = 3 * (size_t)r + kf;
-        rcode[d] = L.rg_code[src];
+        rcode[d] = rgc[src];
         rnx[d] = L.rg_nx[src];
         rny[d] = L.rg_ny[src];
         rsl[d] = L.rg_sL[src];
       }
     std::vector<int32_t> rg_off(L.ntiles, 0);
     for (int t = 1; t < L.ntiles; ++t) rg_off[t] = rg_off[t - 1] + L.ring_n1[t - 1];
-    rc = h.upload(&h.code2, L.nb_slot);
+    rc = h.upload(&h.code2, code2);
     if (!rc) rc = h.upload(&h.ring_off, L.ring_off);
     if (!rc) rc = h.upload(&h.ring_cells, L.ring_cells);
     if (!rc) rc = h.upload(&h.ring_n1, L.ring_n1);
@@ -1233,8 +1300,12 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
     if (!rc) rc = h.upload(&h.rg_nx, rnx);
     if (!rc) rc = h.upload(&h.rg_ny, rny);
     if (!rc) rc = h.upload(&h.rg_sL, rsl);
-    if (!rc) rc = h.upload(&h.rgb_slot, L.rgb_slot);
-    if (!rc) rc = h.upload(&h.rgb_ratio, L.rgb_ratio);
+    if (!rc) rc = h.upload(&h.gh_off, gh_off);
+    if (!rc) rc = h.upload(&h.gh_nown, gh_nown);
+    if (!rc) rc = h.upload(&h.gh_s0, gh_s0);
+    if (!rc) rc = h.upload(&h.gh_s1, gh_s1);
+    if (!rc) rc = h.upload(&h.gh_self, gh_self);
+    if (!rc) rc = h.upload(&h.gh_ratio, gh_ratio);
     if (!rc) rc = h.upload(&h.xnode_off, L.xnode_off);
     if (!rc) rc = h.upload(&h.xnodes, L.xnodes);
     if (!rc) rc = h.upload(&h.xentry_pair_off, L.xentry_pair_off);
@@ -1251,7 +1322,6 @@ static int create_typed(const fvdbm_mesh_desc &m, Handle &h, int reorder) {
     h.pairs = true;
     h.lds2 = lds2;
     h.nrgc = nrgc;
-    h.nrgb = L.nrg;
     h.max_xnodes = L.max_xnodes;
     h.max_xpairs = L.max_xpairs;
   }
