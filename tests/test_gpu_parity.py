@@ -254,6 +254,29 @@ def test_prepare_changes_nothing(gpu):
     _assert_fields_equal(env, st, what="prepared run")
 
 
+def test_launch_mode_default_by_mesh_size(gpu):
+    """By default a handle below 131072 padded cells replays captured graphs
+    (32 steps each) and a larger one launches directly; either mode, set
+    explicitly, gives the same bits as the other after runs that end on
+    pair launches and on a single step."""
+    small = _gpu_env(*_channel(target_cells=8_000, seed=6)[2:])
+    assert small.info()["graph_steps"] == 32
+    small.close()
+    m, dyn, cells, faces, nodes = _channel(target_cells=150_000, seed=2)
+    direct = _gpu_env(cells, faces, nodes)
+    graphs = _gpu_env(cells, faces, nodes, graph_steps=32)
+    assert direct.info()["n_cells_padded"] >= 131072
+    assert direct.info()["graph_steps"] == -1 and graphs.info()["graph_steps"] == 32
+    for k in (40, 33):
+        print(f"launch modes, 150k cells: {k} more steps", flush=True)
+        direct.step(k)
+        graphs.step(k)
+        assert np.array_equal(direct.cells.pdf, graphs.cells.pdf), k
+    assert np.array_equal(direct.nodes.pdf, graphs.nodes.pdf)
+    direct.close()
+    graphs.close()
+
+
 def test_every_cell_written_and_deterministic(gpu):
     """Two runs agree bitwise and every cell holds a finite pdf."""
     m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=2)
