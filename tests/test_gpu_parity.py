@@ -307,6 +307,39 @@ def test_checkpoint_resume_bitwise(gpu, tmp_path):
     _assert_fields_equal(env2, st, what="resumed run")
 
 
+def test_vtk_export_of_a_gpu_state(gpu, tmp_path):
+    """Mesher.to_vtk of a GPU state after 30 channel steps: the exported
+    Velocity, Density, pdf and feq cell data read back to the oracle's
+    float32 values bit for bit (9 significant digits round-trip float32)."""
+    m, dyn, cells, faces, nodes = _channel(target_cells=8_000, seed=4)
+    oenv = oracle_env(cells, faces, nodes)
+    st = oracle_state(oenv, nodes)
+    oracle.run(oenv, st, dyn.tau, dyn.delta_t, 30, "f32")
+    env = _gpu_env(cells, faces, nodes)
+    env.step(30)
+    path = m.to_vtk(env, str(tmp_path / "chan"), save_f=True, save_feq=True)
+    env.close()
+    lines = open(path).read().split("\n")
+    C = len(m.cells)
+
+    def block(header, ncols):
+        i = lines.index(header) + 1
+        if header.startswith("SCALARS"):
+            i += 1  # LOOKUP_TABLE default
+        rows = np.array([[float(v) for v in ln.split()] for ln in lines[i:i + C]])
+        assert rows.shape == (C, ncols), header
+        return rows.astype(np.float32)
+
+    vel = block("VECTORS Velocity double", 3)
+    assert np.array_equal(vel[:, :2], np.asarray(st["cell_vel"]).reshape(C, 2))
+    assert not vel[:, 2].any()
+    assert np.array_equal(block("SCALARS Density double 1", 1).reshape(C),
+                          np.asarray(st["cell_rho"]).reshape(C))
+    assert np.array_equal(block(f"pdf 9 {C} double", 9), st["cell_pdf"])
+    assert np.array_equal(block(f"feq 9 {C} double", 9), np.asarray(st["cell_feq"]).reshape(C, 9))
+    assert "FIELD FieldData 1" in lines
+
+
 def test_close_with_steps_still_queued(gpu):
     """Steps are queued asynchronously; closing a handle right after a long
     queued run lets it finish before its graphs and buffers go. A new handle
