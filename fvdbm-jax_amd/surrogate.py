@@ -22,6 +22,15 @@ weight gradients and the aggregation are this library's HIP kernels
 weights) behind ``torch.autograd.Function``; PyTorch only holds the tensors,
 runs Adam and the loss. There is no CPU path: without a GPU and the built
 library the functions here raise.
+
+Per-case calibration: the network's fields are close in shape, but their
+overall level varies from case to case (the least-squares factor between a
+prediction's rho - 1 and the converged one spreads over 0.7-1.3). A ridge
+model of log(factor) on case-level features (``case_features``: obstacle
+count and sizes, tau, inlet speed, cell count and summaries of the
+prediction itself), fitted on cold-solved cases by ``calibrate_surrogate``,
+rescales rho - 1 and u of each prediction (``apply_calibration``). Weight
+files without the ``calib_*`` arrays predict uncalibrated.
 """
 from __future__ import annotations
 
@@ -38,6 +47,12 @@ IN_PAD = 16
 OUT = 3
 OUT_PAD = 16
 LAYERS = 6
+
+# case-level features of the per-case calibration (``case_features``)
+CALIB_FEATURES = ("obstacles", "solid_fraction", "r_mean", "r_max", "tau", "u_in", "cells",
+                  "pred_inlet_rho", "pred_mean_rho", "pred_umax", "log_cells", "pred_inlet_rho_tau")
+CALIB_KEYS = ("calib_mu", "calib_sd", "calib_w_rho", "calib_b_rho", "calib_w_u", "calib_b_u")
+CALIB_CLIP = (0.5, 2.0)  # bounds of an applied factor
 
 # fixed input scaling: (x - shift) / scale per feature column
 FEATURE_SHIFT = np.array([0.5, 0.25, 0.126, 0, 0, 0, 0, 0.035, 0.725], np.float32)
@@ -452,6 +467,8 @@ class MeshNet:
                        for k, v in p.items() if "." in k}
         self.target_scale = np.asarray(
             params.get("target_scale", target_scale) if params is not None else target_scale, np.float32)
+        self.calib = ({k: np.asarray(params[k], np.float64) for k in CALIB_KEYS}
+                      if params is not None and all(k in params for k in CALIB_KEYS) else None)
 
     def parameters(self):
         return list(self.params.values())
@@ -475,6 +492,8 @@ class MeshNet:
         out["layers"] = np.int32(self.layers)
         out["grid"] = np.array(self.grid, np.int32)
         out["resolution"] = np.int32(self.resolution)
+        if self.calib is not None:
+            out.update(self.calib)
         return out
 
     def save(self, path):
@@ -486,12 +505,14 @@ class MeshNet:
         params = {k: z[k] for k in z.files if k != "record"}
         return cls(params, layers=int(z["layers"]))
 
-    def predict(self, cases, project=False, chunk=64):
+    def predict(self, cases, project=False, chunk=64, calibrate=True):
         """Node (rho, u) of each porous.Case: list of (rho [N], u [N,2]);
-        with ``project``, u_x passes through ``project_mass_flux``. Cases
-        go through the network ``chunk`` at a time; their inputs are
-        prepared by a thread pool (the native builders release the GIL)
-        while the previous chunk runs on the GPU."""
+        with ``project``, u_x passes through ``project_mass_flux``; with
+        ``calibrate`` and calibration arrays in the weights, each case's
+        rho - 1 and u are rescaled by ``apply_calibration``. Cases go
+        through the network ``chunk`` at a time; their inputs are prepared
+        by a thread pool (the native builders release the GIL) while the
+        previous chunk runs on the GPU."""
         torch = _torch()
         import os
         from concurrent.futures import ThreadPoolExecutor
@@ -510,8 +531,48 @@ class MeshNet:
                     o += gi["n"]
                     r = (1.0 + yy[:, 0]).astype(np.float32)
                     v = yy[:, 1:3].astype(np.float32)
+                    if calibrate and self.calib is not None:
+                        r, v = apply_calibration(self.calib, c, r, v)
                     out.append((r, project_mass_flux(c, r, v) if project else v))
         return out
+
+
+def case_features(case, rho, vel):
+    """The CALIB_FEATURES of a porous.Case and its predicted node (rho, u)
+    (speeds and rho - 1 relative to the inlet speed)."""
+    pts = np.asarray(case.mesher.points, np.float64)
+    width = pts[:, 0].max() - pts[:, 0].min()
+    height = pts[:, 1].max() - pts[:, 1].min()
+    d = np.asarray(case.discs, np.float64).reshape(-1, 3)
+    p = np.asarray(rho, np.float64) - 1.0
+    q = np.asarray(vel, np.float64).reshape(-1, 2)
+    inlet = case.mesher.point_markers == porous.meshgen.INLET
+    u_in = float(case.u_in)
+    tau = float(case.dynamics.tau)
+    cells = float(case.n_cells)
+    inlet_rho = float(p[inlet].mean() / u_in) if inlet.any() else 0.0
+    r = d[:, 2] if len(d) else np.zeros(1)
+    return np.array([len(d), np.pi * (d[:, 2] ** 2).sum() / (width * height), r.mean(), r.max(), tau,
+                     u_in, cells, inlet_rho, p.mean() / u_in, np.abs(q).max() / u_in, np.log(cells),
+                     inlet_rho * tau])
+
+
+def calibration_factors(calib, features):
+    """(rho - 1 factor, u factor) of a feature vector: exp of the ridge
+    models on the standardised features, clipped to CALIB_CLIP."""
+    z = (np.asarray(features, np.float64) - calib["calib_mu"]) / calib["calib_sd"]
+    fr = math.exp(float(z @ calib["calib_w_rho"] + calib["calib_b_rho"]))
+    fu = math.exp(float(z @ calib["calib_w_u"] + calib["calib_b_u"]))
+    return (min(max(fr, CALIB_CLIP[0]), CALIB_CLIP[1]), min(max(fu, CALIB_CLIP[0]), CALIB_CLIP[1]))
+
+
+def apply_calibration(calib, case, rho, vel):
+    """rho and u of one case's prediction rescaled by its calibration
+    factors: rho = 1 + f_rho (rho - 1), u = f_u u (float64, stored float32)."""
+    fr, fu = calibration_factors(calib, case_features(case, rho, vel))
+    r = (1.0 + fr * (np.asarray(rho, np.float64) - 1.0)).astype(np.float32)
+    v = (fu * np.asarray(vel, np.float64)).astype(np.float32)
+    return r, v
 
 
 def project_mass_flux(case, rho, vel, bins=64):

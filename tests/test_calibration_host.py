@@ -1,0 +1,64 @@
+"""Per-case calibration of the surrogate, host side: the ridge fit, the
+ideal factors, the case features and how a prediction is rescaled."""
+import importlib
+import math
+
+import numpy as np
+
+cs = importlib.import_module("fvdbm-jax_amd.calibrate_surrogate")
+sg = importlib.import_module("fvdbm-jax_amd.surrogate")
+porous = importlib.import_module("fvdbm-jax_amd.porous")
+
+
+def test_ridge_recovers_a_linear_model():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(400, 5)) * [1, 10, 0.1, 3, 2] + [0, 5, 1, -2, 0]
+    z = (x - x.mean(0)) / x.std(0)
+    w = np.array([0.3, -0.2, 0.0, 0.5, 0.1])
+    y = z @ w + 0.7
+    fit = cs.fit_ridge(x, y, alphas=(1e-6, 1.0, 100.0))
+    assert fit["alpha"] == 1e-6 and fit["r2"] > 0.999
+    assert np.allclose(fit["w"], w, atol=1e-5) and abs(fit["b"] - 0.7) < 1e-9
+    # pure noise: the penalty grows and the fit explains nothing out of fold
+    noise = cs.fit_ridge(x, rng.normal(size=400))
+    assert noise["alpha"] >= 10 and noise["r2"] < 0.05
+
+
+def test_ideal_scales_of_scaled_fields():
+    rng = np.random.default_rng(2)
+    rho_t = 1.0 + 0.01 * rng.normal(size=300)
+    vel_t = 0.05 * rng.normal(size=(300, 2))
+    rho_p = 1.0 + (rho_t - 1.0) / 1.25
+    a_r, a_u = cs.ideal_scales(rho_p, vel_t / 0.8, rho_t, vel_t)
+    assert math.isclose(a_r, 1.25, rel_tol=1e-12) and math.isclose(a_u, 0.8, rel_tol=1e-12)
+
+
+def _calib(b_rho, b_u, nf=len(sg.CALIB_FEATURES)):
+    return dict(calib_mu=np.zeros(nf), calib_sd=np.ones(nf), calib_w_rho=np.zeros(nf),
+                calib_b_rho=np.float64(b_rho), calib_w_u=np.zeros(nf), calib_b_u=np.float64(b_u))
+
+
+def test_case_features_and_rescaling():
+    case = porous.make_case(3, dist=porous.STEADY, target_cells=3000)
+    N = case.n_nodes
+    rng = np.random.default_rng(3)
+    rho = (1.0 + 0.002 * rng.normal(size=N)).astype(np.float32)
+    vel = (case.u_in * rng.normal(size=(N, 2))).astype(np.float32)
+    f = sg.case_features(case, rho, vel)
+    assert f.shape == (len(sg.CALIB_FEATURES),) and np.isfinite(f).all()
+    assert f[0] == len(case.discs) and f[4] == case.dynamics.tau and f[6] == case.n_cells
+    assert math.isclose(f[10], math.log(case.n_cells)) and math.isclose(f[11], f[7] * f[4])
+    r, v = sg.apply_calibration(_calib(math.log(1.5), math.log(0.9)), case, rho, vel)
+    assert r.dtype == np.float32 and v.dtype == np.float32
+    assert np.array_equal(r, (1.0 + 1.5 * (rho.astype(np.float64) - 1.0)).astype(np.float32))
+    assert np.array_equal(v, (0.9 * vel.astype(np.float64)).astype(np.float32))
+    # factors outside the clip range are held at its ends
+    assert sg.calibration_factors(_calib(math.log(5.0), math.log(0.1)), f) == (2.0, 0.5)
+
+
+def test_shipped_weights_carry_a_calibration():
+    z = np.load(str(porous.__file__).replace("porous.py", "data/surrogate_porous.npz"))
+    assert all(k in z.files for k in sg.CALIB_KEYS)
+    nf = len(sg.CALIB_FEATURES)
+    assert z["calib_mu"].shape == (nf,) and z["calib_w_rho"].shape == (nf,)
+    assert (z["calib_sd"] > 0).all()

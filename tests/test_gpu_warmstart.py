@@ -72,3 +72,24 @@ def test_shipped_network_gains_over_cold(gpu):
     ratio = cold["steps"] / warm["steps"]
     assert np.median(ratio) > 1.3, ratio
     assert warm["steps"].sum() < cold["steps"].sum()
+
+
+def test_calibrated_prediction_rescales_each_case(gpu):
+    """The shipped weights carry the per-case calibration: each case's
+    calibrated rho and u are its raw prediction rescaled by the factors of
+    its own case features, bit for bit."""
+    sg = importlib.import_module("fvdbm-jax_amd.surrogate")
+    net = sg.MeshNet.load(WEIGHTS)
+    assert net.calib is not None
+    cases = [fv.porous.make_case(7_000_000 + k, dist=fv.porous.STEADY, target_cells=20_000)
+             for k in range(3)]
+    raw = net.predict(cases, calibrate=False)
+    cal = net.predict(cases)
+    factors = []
+    for c, (r0, v0), (r1, v1) in zip(cases, raw, cal):
+        fr, fu = sg.calibration_factors(net.calib, sg.case_features(c, r0, v0))
+        factors.append(fr)
+        assert np.array_equal(r1, (1.0 + fr * (r0.astype(np.float64) - 1.0)).astype(np.float32))
+        assert np.array_equal(v1, (fu * v0.astype(np.float64)).astype(np.float32))
+    assert all(sg.CALIB_CLIP[0] <= f <= sg.CALIB_CLIP[1] for f in factors)
+    assert len(set(factors)) == len(factors)  # a factor per case
