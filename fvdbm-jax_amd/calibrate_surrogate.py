@@ -71,13 +71,23 @@ def fit_ridge(x, y, alphas=ALPHAS, folds=5, seed=0):
             best = (a, sse)
     a = best[0]
     w, b = _ridge(z, y, a)
-    r2 = 1.0 - best[1] / float(((y - y.mean()) ** 2).sum())
+    ss = float(((y - y.mean()) ** 2).sum())
+    r2 = 1.0 - best[1] / ss if ss > 0 else 0.0  # a constant target: nothing to explain
     return dict(mu=mu, sd=sd, w=w, b=b, alpha=a, r2=r2)
+
+
+MIN_CASES = 20
 
 
 def calibration_arrays(features, a_rho, a_u):
     """The ``calib_*`` arrays (and a record) from per-case features and
-    ideal factors."""
+    ideal factors (at least MIN_CASES cases, every factor positive)."""
+    a_rho = np.asarray(a_rho, np.float64)
+    a_u = np.asarray(a_u, np.float64)
+    if len(a_rho) < MIN_CASES:
+        raise ValueError(f"calibration needs at least {MIN_CASES} converged cases, got {len(a_rho)}")
+    if not ((a_rho > 0).all() and (a_u > 0).all() and np.isfinite(features).all()):
+        raise ValueError("calibration needs positive ideal factors and finite features")
     fr = fit_ridge(features, np.log(a_rho))
     fu = fit_ridge(features, np.log(a_u))
     arrays = dict(calib_mu=fr["mu"], calib_sd=fr["sd"], calib_w_rho=fr["w"],

@@ -559,8 +559,11 @@ def case_features(case, rho, vel):
 
 def calibration_factors(calib, features):
     """(rho - 1 factor, u factor) of a feature vector: exp of the ridge
-    models on the standardised features, clipped to CALIB_CLIP."""
+    models on the standardised features, clipped to CALIB_CLIP; (1, 1) for
+    a feature vector that is not finite."""
     z = (np.asarray(features, np.float64) - calib["calib_mu"]) / calib["calib_sd"]
+    if not np.isfinite(z).all():
+        return 1.0, 1.0  # a non-finite prediction is left as it is (the warm start rejects it)
     fr = math.exp(float(z @ calib["calib_w_rho"] + calib["calib_b_rho"]))
     fu = math.exp(float(z @ calib["calib_w_u"] + calib["calib_b_u"]))
     return (min(max(fr, CALIB_CLIP[0]), CALIB_CLIP[1]), min(max(fu, CALIB_CLIP[0]), CALIB_CLIP[1]))

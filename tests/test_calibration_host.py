@@ -62,3 +62,18 @@ def test_shipped_weights_carry_a_calibration():
     nf = len(sg.CALIB_FEATURES)
     assert z["calib_mu"].shape == (nf,) and z["calib_w_rho"].shape == (nf,)
     assert (z["calib_sd"] > 0).all()
+
+
+def test_calibration_guards():
+    nf = len(sg.CALIB_FEATURES)
+    f = np.full(nf, np.nan)
+    assert sg.calibration_factors(_calib(math.log(1.5), 0.0), f) == (1.0, 1.0)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(30, nf))
+    arrays, record = cs.calibration_arrays(x, np.exp(0.1 * rng.normal(size=30)), np.ones(30))
+    assert set(arrays) == set(sg.CALIB_KEYS) and record["cases"] == 30
+    import pytest
+    with pytest.raises(ValueError):
+        cs.calibration_arrays(x[:5], np.ones(5), np.ones(5))
+    with pytest.raises(ValueError):
+        cs.calibration_arrays(x, -np.ones(30), np.ones(30))
